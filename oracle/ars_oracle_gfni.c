@@ -1,0 +1,195 @@
+/*
+ * ars_oracle_gfni.c -- CPU ORACLE, AVX-512BW + GFNI restatement of the
+ * reference's native SIMD kernel (rs/src/kernel/gfni_kernel.rs).
+ *
+ * TEST INFRASTRUCTURE ONLY.  Used (a) as the CPU baseline that bench.py times
+ * beside the GPU path, and (b) in tests, where it must equal the LUT
+ * restatement in ars_oracle.c byte for byte (the reference's own check,
+ * rs/src/kernel/gfni_kernel.rs:324-387).  It is never linked into the product.
+ *
+ * The codec glue (which transforms run, in which order) is the same as in
+ * ars_oracle.c (rs/src/codec.rs:241-265, 479-569); only the butterfly and
+ * scale loops change.  Zero twiddles take the XOR-only path that the
+ * reference's generated unrolled kernels take (rs/build.rs:36-46, 86-96).
+ */
+#include <stdint.h>
+#include <stddef.h>
+#include <string.h>
+#include <immintrin.h>
+
+void aro_tables(uint8_t *exp, uint8_t *log, uint8_t *inv, uint8_t *basis,
+                uint8_t *subspace, uint8_t *spv, uint8_t *spcoeff, uint64_t *gfni);
+uint8_t aro_mul_lut(uint8_t a, uint8_t b);
+
+static uint8_t G_BASIS[8], G_SUBSPACE[256], G_SPV[9][256], G_INV[256];
+static uint64_t G_GFNI[256];
+static int g_ready = 0;
+
+static void g_init(void) {
+    if (g_ready) return;
+    aro_tables(NULL, NULL, G_INV, G_BASIS, G_SUBSPACE, &G_SPV[0][0], NULL, G_GFNI);
+    g_ready = 1;
+}
+
+int aro_gfni_supported(void) {
+    __builtin_cpu_init();
+    return __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512bw") &&
+           __builtin_cpu_supports("gfni");
+}
+
+#define TGT __attribute__((target("avx512f,avx512bw,gfni")))
+
+/* rs/src/kernel/gfni_kernel.rs:14-44: (a, b) <- (a + t*b, b + a + t*b). */
+TGT static void bfly_fwd(uint8_t *a, uint8_t *b, size_t len, __m512i mat, int zero) {
+    size_t i = 0;
+    for (; i + 64 <= len; i += 64) {
+        __m512i va = _mm512_loadu_si512((const void *)(a + i));
+        __m512i vb = _mm512_loadu_si512((const void *)(b + i));
+        if (!zero) va = _mm512_xor_si512(va, _mm512_gf2p8affine_epi64_epi8(vb, mat, 0));
+        vb = _mm512_xor_si512(vb, va);
+        if (!zero) _mm512_storeu_si512((void *)(a + i), va);
+        _mm512_storeu_si512((void *)(b + i), vb);
+    }
+    if (i < len) {
+        __mmask64 k = (1ULL << (len - i)) - 1;
+        __m512i va = _mm512_maskz_loadu_epi8(k, a + i);
+        __m512i vb = _mm512_maskz_loadu_epi8(k, b + i);
+        if (!zero) va = _mm512_xor_si512(va, _mm512_gf2p8affine_epi64_epi8(vb, mat, 0));
+        vb = _mm512_xor_si512(vb, va);
+        _mm512_mask_storeu_epi8(a + i, k, va);
+        _mm512_mask_storeu_epi8(b + i, k, vb);
+    }
+}
+
+/* rs/src/kernel/gfni_kernel.rs:46-76: (g0, g1) <- (g0 + t*(g0+g1), g0+g1). */
+TGT static void bfly_inv(uint8_t *a, uint8_t *b, size_t len, __m512i mat, int zero) {
+    size_t i = 0;
+    for (; i + 64 <= len; i += 64) {
+        __m512i va = _mm512_loadu_si512((const void *)(a + i));
+        __m512i vb = _mm512_loadu_si512((const void *)(b + i));
+        vb = _mm512_xor_si512(vb, va);
+        if (!zero) va = _mm512_xor_si512(va, _mm512_gf2p8affine_epi64_epi8(vb, mat, 0));
+        if (!zero) _mm512_storeu_si512((void *)(a + i), va);
+        _mm512_storeu_si512((void *)(b + i), vb);
+    }
+    if (i < len) {
+        __mmask64 k = (1ULL << (len - i)) - 1;
+        __m512i va = _mm512_maskz_loadu_epi8(k, a + i);
+        __m512i vb = _mm512_maskz_loadu_epi8(k, b + i);
+        vb = _mm512_xor_si512(vb, va);
+        if (!zero) va = _mm512_xor_si512(va, _mm512_gf2p8affine_epi64_epi8(vb, mat, 0));
+        _mm512_mask_storeu_epi8(a + i, k, va);
+        _mm512_mask_storeu_epi8(b + i, k, vb);
+    }
+}
+
+/* rs/src/kernel/gfni_kernel.rs:78-107 */
+TGT void aro_gfni_fft_sharded(uint8_t *shards, size_t L, int k, uint8_t beta) {
+    if (k == 0) return;
+    g_init();
+    size_t half = (size_t)1 << (k - 1);
+    uint8_t tw = G_SPV[k - 1][beta];
+    __m512i mat = _mm512_set1_epi64((long long)G_GFNI[tw]);
+    for (size_t i = 0; i < half; i++) bfly_fwd(shards + i * L, shards + (i + half) * L, L, mat, tw == 0);
+    aro_gfni_fft_sharded(shards, L, k - 1, beta);
+    aro_gfni_fft_sharded(shards + half * L, L, k - 1, beta ^ G_BASIS[k - 1]);
+}
+
+/* rs/src/kernel/gfni_kernel.rs:109-150 */
+TGT void aro_gfni_ifft_sharded(uint8_t *shards, size_t L, int k, uint8_t beta) {
+    if (k == 0) return;
+    g_init();
+    size_t half = (size_t)1 << (k - 1);
+    aro_gfni_ifft_sharded(shards, L, k - 1, beta);
+    aro_gfni_ifft_sharded(shards + half * L, L, k - 1, beta ^ G_BASIS[k - 1]);
+    uint8_t tw = G_SPV[k - 1][beta];
+    __m512i mat = _mm512_set1_epi64((long long)G_GFNI[tw]);
+    for (size_t i = 0; i < half; i++) bfly_inv(shards + i * L, shards + (i + half) * L, L, mat, tw == 0);
+}
+
+/* rs/src/kernel/gfni_kernel.rs:152-173 */
+TGT static void scale_gfni(const uint8_t *src, uint8_t *dst, size_t len, uint8_t c) {
+    __m512i mat = _mm512_set1_epi64((long long)G_GFNI[c]);
+    size_t i = 0;
+    for (; i + 64 <= len; i += 64)
+        _mm512_storeu_si512((void *)(dst + i),
+                            _mm512_gf2p8affine_epi64_epi8(_mm512_loadu_si512((const void *)(src + i)), mat, 0));
+    if (i < len) {
+        __mmask64 k = (1ULL << (len - i)) - 1;
+        _mm512_mask_storeu_epi8(dst + i, k,
+                                _mm512_gf2p8affine_epi64_epi8(_mm512_maskz_loadu_epi8(k, src + i), mat, 0));
+    }
+}
+
+static int is_pow2(size_t v) { return v && !(v & (v - 1)); }
+static int ilog2(size_t v) { int r = 0; while ((size_t)1 << r < v) r++; return r; }
+
+/* rs/src/codec.rs:241-265 over the GFNI kernel. */
+TGT int aro_gfni_encode_systematic_sharded(size_t n, size_t T, const uint8_t *message,
+                                           uint8_t *parity, uint8_t *workspace, size_t L) {
+    g_init();
+    if (!(n >= 2 && n <= 256 && is_pow2(n) && T >= 1 && T < n && is_pow2(T))) return -1;
+    int t_log = ilog2(T);
+    size_t k = n - T;
+    for (size_t i = 0; i < k / T; i++) {
+        memcpy(workspace, message + i * T * L, T * L);
+        aro_gfni_ifft_sharded(workspace, L, t_log, G_SUBSPACE[(i + 1) * T]);
+        size_t j = 0, len = T * L;
+        for (; j + 64 <= len; j += 64)
+            _mm512_storeu_si512((void *)(parity + j),
+                                _mm512_xor_si512(_mm512_loadu_si512((const void *)(parity + j)),
+                                                 _mm512_loadu_si512((const void *)(workspace + j))));
+        for (; j < len; j++) parity[j] ^= workspace[j];
+    }
+    aro_gfni_fft_sharded(parity, L, t_log, 0);
+    return 0;
+}
+
+/* rs/src/codec.rs:479-569 over the GFNI kernel (lambda / denominators as in
+ * rs/src/codec.rs:436-465). */
+TGT int aro_gfni_recover_erasure_shards(size_t n, size_t T, uint8_t *received, uint8_t *workspace,
+                                        size_t L, const uint8_t *pos, size_t e) {
+    g_init();
+    if (!(n >= 2 && n <= 256 && is_pow2(n) && T >= 1 && T < n && is_pow2(T))) return -1;
+    if (e > T) return 0;
+    if (e == 0) return 1;
+    for (size_t i = 0; i < e; i++) if (pos[i] >= n) return -2;
+    int t_log = ilog2(T), n_log = ilog2(n);
+    uint8_t pts[256], lambda[257], denoms[256];
+    for (size_t i = 0; i < e; i++) pts[i] = G_SUBSPACE[pos[i]];
+    memset(lambda, 0, sizeof lambda);
+    lambda[0] = 1;
+    for (size_t k = 0; k < e; k++) {
+        for (size_t j = k + 1; j >= 1; j--) lambda[j] = lambda[j - 1] ^ aro_mul_lut(pts[k], lambda[j]);
+        lambda[0] = aro_mul_lut(pts[k], lambda[0]);
+    }
+    for (size_t j = 0; j < e; j++) {
+        uint8_t acc = 1;
+        for (size_t i = 0; i < e; i++) if (i != j) acc = aro_mul_lut(acc, pts[j] ^ pts[i]);
+        denoms[j] = acc;
+    }
+    memcpy(workspace, received, T * L);
+    aro_gfni_ifft_sharded(workspace, L, t_log, 0);
+    for (size_t c = 1; c < n / T; c++) {
+        uint8_t *tmp = workspace + T * L;
+        memcpy(tmp, received + c * T * L, T * L);
+        aro_gfni_ifft_sharded(tmp, L, t_log, G_SUBSPACE[c * T]);
+        for (size_t j = 0; j < T * L; j++) workspace[j] ^= tmp[j];
+    }
+    memset(workspace + T * L, 0, (n - T) * L);
+    uint8_t lev[256];
+    for (size_t i = 0; i < n; i++) {
+        uint8_t p = G_SUBSPACE[i], v = lambda[e];
+        for (size_t j = e; j-- > 0;) v = aro_mul_lut(v, p) ^ lambda[j];
+        lev[i] = v;
+    }
+    aro_gfni_fft_sharded(workspace, L, n_log, 0);
+    for (size_t i = 0; i < n; i++) scale_gfni(workspace + i * L, workspace + i * L, L, lev[i]);
+    aro_gfni_ifft_sharded(workspace, L, n_log, 0);
+    memmove(workspace, workspace + T * L, e * L);
+    memset(workspace + e * L, 0, (n - e) * L);
+    aro_gfni_fft_sharded(workspace, L, n_log, 0);
+    for (size_t i = 0; i < e; i++)
+        scale_gfni(workspace + (size_t)pos[i] * L, received + (size_t)pos[i] * L, L, G_INV[denoms[i]]);
+    return 1;
+}
