@@ -1,0 +1,221 @@
+"""Python host mirror of the MI355X sharded additive-FFT Reed-Solomon codec.
+
+The product is the C-ABI library libars.so (include/ars.h); this module binds
+it with ctypes and mirrors the reference crate's two hot-path methods:
+
+    Codec::encode_systematic_sharded(message, parity, workspace, shard_len)
+        (rs/src/codec.rs:241-265)
+    Codec::recover_erasure_shards(received, workspace, shard_len, positions) -> bool
+        (rs/src/codec.rs:479-569)
+
+Buffers are torch uint8 CUDA tensors (device pointers, the HBM path) or
+numpy uint8 arrays (host slices, staged through device memory).  There is no
+CPU fallback: if libars.so is missing or no GPU is present the calls raise.
+
+The directory name contains hyphens, so load it with `load()` below (or
+importlib) rather than a plain import statement.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libars.so")
+
+ARS_OK = 0
+ARS_TRUE = 1
+ARS_FALSE = 0
+ARS_E_INVAL = -1
+ARS_E_POSITION = -2
+ARS_E_DUPLICATE = -3
+ARS_E_NULL = -4
+ARS_E_HIP = -5
+ARS_E_NOMEM = -6
+ARS_E_UNSUPPORTED = -7
+ARS_E_BUDGET = -8
+ARS_E_STATE = -9
+ARS_ENC_ACCUMULATE = 1
+
+_ERRNAMES = {v: k for k, v in dict(globals()).items() if k.startswith("ARS_E_")}
+
+# Every symbol include/ars.h declares, with its ctypes signature.
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_sz = ctypes.c_size_t
+_i = ctypes.c_int
+SIGNATURES = {
+    "ars_create": (_i, [_i, ctypes.POINTER(_vp)]),
+    "ars_destroy": (_i, [_vp]),
+    "ars_last_error": (ctypes.c_char_p, []),
+    "ars_version": (ctypes.c_char_p, []),
+    "ars_encode": (_i, [_vp, _u32, _u32, _vp, _vp, _sz, _u32, _vp]),
+    "ars_recover": (_i, [_vp, _u32, _u32, _vp, _sz, _vp, _u32, _vp]),
+    "ars_recompute_data": (_i, [_vp, _u32, _u32, _vp, _sz, _vp]),
+    "ars_recover_plan_size": (_sz, []),
+    "ars_recover_plan": (_i, [_vp, _u32, _u32, _vp, _u32, _vp, _vp]),
+    "ars_recover_with_plan": (_i, [_vp, _u32, _u32, _vp, _sz, _vp, _vp]),
+    "ars_encode_host": (_i, [_vp, _u32, _u32, _vp, _vp, _sz, _u32]),
+    "ars_recover_host": (_i, [_vp, _u32, _u32, _vp, _sz, _vp, _u32]),
+}
+
+_lib = None
+
+
+class ArsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERRNAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libars.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built; run additive-fft-reed-solomon_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise ArsError(rc, lib().ars_last_error().decode())
+    return rc
+
+
+def _ptr(buf):
+    """Device or host address of a torch tensor / numpy array."""
+    if hasattr(buf, "data_ptr"):
+        if buf.dtype.__repr__() != "torch.uint8":
+            raise TypeError("tensor must be torch.uint8")
+        if not buf.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return buf.data_ptr(), buf.is_cuda
+    a = buf
+    if not isinstance(a, np.ndarray) or a.dtype != np.uint8 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("array must be a C-contiguous numpy uint8 array")
+    return a.ctypes.data, False
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        try:
+            import torch
+            return torch.cuda.current_stream().cuda_stream
+        except Exception:
+            return None
+    return getattr(stream, "cuda_stream", stream)
+
+
+class Context:
+    """Per-device context: resident twiddle and field tables."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().ars_create(device, ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            lib().ars_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # raw C-ABI calls ---------------------------------------------------
+    def encode(self, n, k, d_message, d_parity, shard_len, flags=0, stream=None):
+        return _check(lib().ars_encode(self.handle, n, k, d_message, d_parity, shard_len, flags,
+                                       _stream_ptr(stream)))
+
+    def recover(self, n, k, d_received, shard_len, positions, stream=None):
+        pos = np.ascontiguousarray(np.asarray(positions, dtype=np.uint8))
+        return _check(lib().ars_recover(self.handle, n, k, d_received, shard_len,
+                                        pos.ctypes.data if len(pos) else None, len(pos),
+                                        _stream_ptr(stream)))
+
+    def recover_plan(self, n, k, positions, d_plan, stream=None):
+        pos = np.ascontiguousarray(np.asarray(positions, dtype=np.uint8))
+        return _check(lib().ars_recover_plan(self.handle, n, k, pos.ctypes.data if len(pos) else None,
+                                             len(pos), d_plan, _stream_ptr(stream)))
+
+    def recover_with_plan(self, n, k, d_received, shard_len, d_plan, stream=None):
+        return _check(lib().ars_recover_with_plan(self.handle, n, k, d_received, shard_len, d_plan,
+                                                  _stream_ptr(stream)))
+
+    def recompute_data(self, n, k, d_received, shard_len, stream=None):
+        return _check(lib().ars_recompute_data(self.handle, n, k, d_received, shard_len,
+                                               _stream_ptr(stream)))
+
+
+_default_ctx = {}
+
+
+def default_context(device=0):
+    if device not in _default_ctx:
+        _default_ctx[device] = Context(device)
+    return _default_ctx[device]
+
+
+class Codec:
+    """Mirror of the reference's `Codec<G, B, K, N, T>` / `Rs<N, T>` with (n, k)
+    at run time.  Shard layout: parity shards 0..T-1, message T..n-1."""
+
+    def __init__(self, n, k, device=0):
+        self.n, self.k, self.t = n, k, n - k
+        self.ctx = default_context(device)
+
+    def encode_systematic_sharded(self, message, parity, workspace=None, shard_len=None,
+                                  accumulate=True, stream=None):
+        """rs/src/codec.rs:241-265.  As in the reference, parity is
+        XOR-accumulated into (pass zeroed parity for a plain encode, or
+        accumulate=False to overwrite).  `workspace` is accepted for signature
+        parity and unused (tiles live in LDS)."""
+        if shard_len is None:
+            shard_len = len(message) // self.k
+        flags = ARS_ENC_ACCUMULATE if accumulate else 0
+        mp, mdev = _ptr(message)
+        pp, pdev = _ptr(parity)
+        if len(message) != self.k * shard_len or len(parity) != self.t * shard_len:
+            raise ValueError("buffer sizes do not match (n, k, shard_len)")
+        if mdev and pdev:
+            return self.ctx.encode(self.n, self.k, mp, pp, shard_len, flags, stream)
+        if mdev or pdev:
+            raise ValueError("message and parity must both be device or both host buffers")
+        return _check(lib().ars_encode_host(self.ctx.handle, self.n, self.k, mp, pp, shard_len, flags))
+
+    def recover_erasure_shards(self, received, workspace=None, shard_len=None,
+                               erasure_positions=(), stream=None):
+        """rs/src/codec.rs:479-569.  Returns the reference's bool."""
+        if shard_len is None:
+            shard_len = len(received) // self.n
+        if len(received) != self.n * shard_len:
+            raise ValueError("received size does not match (n, shard_len)")
+        rp, dev = _ptr(received)
+        pos = np.ascontiguousarray(np.asarray(erasure_positions, dtype=np.uint8))
+        if dev:
+            rc = self.ctx.recover(self.n, self.k, rp, shard_len, pos, stream)
+        else:
+            rc = _check(lib().ars_recover_host(self.ctx.handle, self.n, self.k, rp, shard_len,
+                                               pos.ctypes.data if len(pos) else None, len(pos)))
+        return rc == ARS_TRUE
+
+    def recompute_data_from_parity_sharded(self, received, workspace=None, shard_len=None,
+                                           stream=None):
+        """rs/src/codec.rs:321-340 (n = 2T); device buffers only."""
+        if shard_len is None:
+            shard_len = len(received) // self.n
+        rp, dev = _ptr(received)
+        if not dev:
+            raise ValueError("recompute_data_from_parity_sharded takes a device buffer")
+        return self.ctx.recompute_data(self.n, self.k, rp, shard_len, stream)

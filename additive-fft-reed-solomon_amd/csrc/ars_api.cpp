@@ -1,0 +1,259 @@
+// ars_api.cpp -- the C ABI (include/ars.h): argument validation, resident
+// tables, and kernel dispatch.  No CPU fallback exists: every entry point
+// runs the HIP kernels or returns an error.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+
+#include "../../include/ars.h"
+#include "ars_gf.hpp"
+#include "ars_kernels.hpp"
+
+struct ars_ctx {
+    int device = 0;
+    ars::MulTab *d_tw = nullptr;  // 255 twiddle multipliers, resident
+    uint8_t *d_gtab = nullptr;    // exp / log / omega for the recover prep kernel
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *msg) {
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *where) {
+    g_err = std::string(where) + ": " + hipGetErrorString(e);
+    return ARS_E_HIP;
+}
+
+bool is_pow2(uint32_t v) { return v && !(v & (v - 1)); }
+int ilog2(uint32_t v) {
+    int r = 0;
+    while ((1u << r) < v) ++r;
+    return r;
+}
+
+int check_code(uint32_t n, uint32_t k, size_t L) {
+    if (n < 2 || n > 256 || !is_pow2(n)) return fail(ARS_E_INVAL, "n must be a power of two in [2, 256]");
+    if (k < 1 || k >= n) return fail(ARS_E_INVAL, "k must satisfy 1 <= k < n");
+    if (L == 0) return fail(ARS_E_INVAL, "shard_len must be >= 1");
+    const uint32_t T = n - k;
+    if (!is_pow2(T) && !is_pow2(k))
+        return fail(ARS_E_UNSUPPORTED, "neither T = n - k nor k is a power of two");
+    return ARS_OK;
+}
+
+bool aligned4(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+const char *ars_last_error(void) { return g_err.c_str(); }
+const char *ars_version(void) { return "ars-mi355x 0.1 (gfx950)"; }
+
+int ars_create(int device, ars_ctx **out) {
+    if (!out) return fail(ARS_E_NULL, "out is null");
+    *out = nullptr;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    ars_ctx *c = new (std::nothrow) ars_ctx;
+    if (!c) return fail(ARS_E_NOMEM, "context allocation failed");
+    c->device = device;
+    const ars::GfTables &g = ars::gf();
+    ars::MulTab tw[ars::kTwiddleEntries];
+    for (int l = 0; l < 8; ++l)
+        for (int j = 0; j < (128 >> l); ++j)
+            tw[ars::lvl_off(l) + j] = g.multab(g.spv[l][g.sub[j << (l + 1)]]);
+    uint8_t gtab[ars::kGtabBytes];
+    std::memcpy(gtab, g.exp, 512);
+    std::memcpy(gtab + 512, g.log, 256);
+    std::memcpy(gtab + 768, g.sub, 256);
+    if ((e = hipMalloc(&c->d_tw, sizeof tw)) != hipSuccess ||
+        (e = hipMalloc(&c->d_gtab, sizeof gtab)) != hipSuccess ||
+        (e = hipMemcpy(c->d_tw, tw, sizeof tw, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(c->d_gtab, gtab, sizeof gtab, hipMemcpyHostToDevice)) != hipSuccess) {
+        hipFree(c->d_tw);
+        hipFree(c->d_gtab);
+        delete c;
+        return hip_fail(e, "context tables");
+    }
+    *out = c;
+    return ARS_OK;
+}
+
+int ars_destroy(ars_ctx *ctx) {
+    if (!ctx) return ARS_OK;
+    hipSetDevice(ctx->device);
+    hipFree(ctx->d_tw);
+    hipFree(ctx->d_gtab);
+    delete ctx;
+    return ARS_OK;
+}
+
+int ars_encode(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *d_message, uint8_t *d_parity,
+               size_t L, uint32_t flags, void *stream) {
+    if (!ctx || !d_message || !d_parity) return fail(ARS_E_NULL, "null context or buffer");
+    int rc = check_code(n, k, L);
+    if (rc) return rc;
+    if (flags & ~ARS_ENC_ACCUMULATE) return fail(ARS_E_INVAL, "unknown encode flags");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint32_t T = n - k;
+    const bool al = (L % 4 == 0) && aligned4(d_message) && aligned4(d_parity);
+    if (is_pow2(T)) {
+        e = ars::launch_encode_ref(ilog2(T), al, (flags & ARS_ENC_ACCUMULATE) != 0, d_message, d_parity,
+                                   L, k / T, ctx->d_tw, s);
+    } else {
+        if (flags & ARS_ENC_ACCUMULATE)
+            return fail(ARS_E_UNSUPPORTED, "ACCUMULATE is defined only when T is a power of two");
+        e = ars::launch_encode_kpow2(ilog2(k), al, d_message, d_parity, L, n / k - 1, n - k,
+                                     ctx->d_tw, s);
+    }
+    if (e != hipSuccess) return hip_fail(e, "encode launch");
+    return ARS_OK;
+}
+
+namespace {
+int validate_positions(uint32_t n, const uint8_t *positions, uint32_t e_cnt, ars::PosArg &pa) {
+    if (!positions) return fail(ARS_E_NULL, "positions is null");
+    bool seen[256] = {false};
+    for (uint32_t i = 0; i < e_cnt; ++i) {
+        const uint8_t p = positions[i];
+        if (p >= n) return fail(ARS_E_POSITION, "erasure position >= n");
+        if (seen[p]) return fail(ARS_E_DUPLICATE, "duplicate erasure position");
+        seen[p] = true;
+        pa.p[i] = p;
+    }
+    return ARS_OK;
+}
+}  // namespace
+
+size_t ars_recover_plan_size(void) { return sizeof(ars::RecPrep); }
+
+int ars_recover_plan(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *positions, uint32_t e_cnt,
+                     void *d_plan, void *stream) {
+    if (!ctx || !d_plan) return fail(ARS_E_NULL, "null context or plan");
+    int rc = check_code(n, k, 1);
+    if (rc) return rc;
+    if (e_cnt > n - k) return ARS_FALSE;
+    ars::PosArg pa;
+    if (e_cnt && (rc = validate_positions(n, positions, e_cnt, pa))) return rc;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    e = ars::launch_recover_prep(static_cast<ars::RecPrep *>(d_plan), n, e_cnt, pa, ctx->d_gtab,
+                                 static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "recover plan launch");
+    return ARS_TRUE;
+}
+
+int ars_recover_with_plan(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_t L,
+                          const void *d_plan, void *stream) {
+    if (!ctx || !d_received || !d_plan) return fail(ARS_E_NULL, "null context, buffer or plan");
+    int rc = check_code(n, k, L);
+    if (rc) return rc;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const bool al = (L % 4 == 0) && aligned4(d_received);
+    e = ars::launch_recover(ilog2(n), al, d_received, L, static_cast<const ars::RecPrep *>(d_plan),
+                            ctx->d_tw, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "recover launch");
+    return ARS_TRUE;
+}
+
+int ars_recover(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_t L,
+                const uint8_t *positions, uint32_t e_cnt, void *stream) {
+    if (!ctx || !d_received) return fail(ARS_E_NULL, "null context or buffer");
+    int rc = check_code(n, k, L);
+    if (rc) return rc;
+    const uint32_t T = n - k;
+    // rs/src/codec.rs:491-496: too many erasures -> false, none -> true.
+    if (e_cnt > T) return ARS_FALSE;
+    if (e_cnt == 0) return ARS_TRUE;
+    ars::PosArg pa;
+    if ((rc = validate_positions(n, positions, e_cnt, pa))) return rc;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ars::RecPrep *prep = nullptr;
+    if ((e = hipMallocAsync(reinterpret_cast<void **>(&prep), sizeof(ars::RecPrep), s)) != hipSuccess)
+        return hip_fail(e, "hipMallocAsync");
+    const bool al = (L % 4 == 0) && aligned4(d_received);
+    e = ars::launch_recover_prep(prep, n, e_cnt, pa, ctx->d_gtab, s);
+    if (e == hipSuccess) e = ars::launch_recover(ilog2(n), al, d_received, L, prep, ctx->d_tw, s);
+    hipError_t ef = hipFreeAsync(prep, s);
+    if (e != hipSuccess) return hip_fail(e, "recover launch");
+    if (ef != hipSuccess) return hip_fail(ef, "hipFreeAsync");
+    return ARS_TRUE;
+}
+
+int ars_recompute_data(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_t L,
+                       void *stream) {
+    if (!ctx || !d_received) return fail(ARS_E_NULL, "null context or buffer");
+    int rc = check_code(n, k, L);
+    if (rc) return rc;
+    if (n != 2 * (n - k)) return fail(ARS_E_INVAL, "recompute_data requires n = 2T");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const bool al = (L % 4 == 0) && aligned4(d_received);
+    e = ars::launch_recompute(ilog2(n - k), al, d_received, L, ctx->d_tw, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "recompute launch");
+    return ARS_OK;
+}
+
+int ars_encode_host(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *message, uint8_t *parity,
+                    size_t L, uint32_t flags) {
+    if (!ctx || !message || !parity) return fail(ARS_E_NULL, "null context or buffer");
+    int rc = check_code(n, k, L);
+    if (rc) return rc;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const size_t mb = (size_t)k * L, pb = (size_t)(n - k) * L;
+    uint8_t *dm = nullptr, *dp = nullptr;
+    if ((e = hipMalloc(&dm, mb)) != hipSuccess || (e = hipMalloc(&dp, pb)) != hipSuccess) {
+        hipFree(dm);
+        return hip_fail(e, "hipMalloc");
+    }
+    e = hipMemcpy(dm, message, mb, hipMemcpyHostToDevice);
+    if (e == hipSuccess && (flags & ARS_ENC_ACCUMULATE)) e = hipMemcpy(dp, parity, pb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        rc = ars_encode(ctx, n, k, dm, dp, L, flags, nullptr);
+        if (rc == ARS_OK) e = hipMemcpy(parity, dp, pb, hipMemcpyDeviceToHost);
+    }
+    hipFree(dm);
+    hipFree(dp);
+    if (e != hipSuccess) return hip_fail(e, "encode_host");
+    return rc;
+}
+
+int ars_recover_host(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *received, size_t L,
+                     const uint8_t *positions, uint32_t e_cnt) {
+    if (!ctx || !received) return fail(ARS_E_NULL, "null context or buffer");
+    int rc = check_code(n, k, L);
+    if (rc) return rc;
+    if (e_cnt > n - k) return ARS_FALSE;
+    if (e_cnt == 0) return ARS_TRUE;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const size_t b = (size_t)n * L;
+    uint8_t *d = nullptr;
+    if ((e = hipMalloc(&d, b)) != hipSuccess) return hip_fail(e, "hipMalloc");
+    e = hipMemcpy(d, received, b, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        rc = ars_recover(ctx, n, k, d, L, positions, e_cnt, nullptr);
+        if (rc == ARS_TRUE) e = hipMemcpy(received, d, b, hipMemcpyDeviceToHost);
+    }
+    hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "recover_host");
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,35 @@
+// ars_kernels.hpp -- launch interface between the C-ABI layer and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "ars_gf.hpp"
+
+namespace ars {
+
+// Per-recover-job multiplier tables, computed on the device (k_recover_prep).
+struct RecPrep {
+    MulTab tab[256];      // erased shard: 1 / Pi'(omega_i); other shards: Pi(omega_i)
+    uint8_t erased[256];  // 1 for erased shards
+};
+
+struct PosArg {
+    uint8_t p[256];
+};
+
+// Device copy of the field tables used by k_recover_prep:
+// exp[0..511], log[512..767], omega[768..1023].
+constexpr int kGtabBytes = 1024;
+
+hipError_t launch_encode_ref(int tl, bool aligned, bool accumulate, const uint8_t *msg, uint8_t *par,
+                             uint64_t L, uint32_t chunks, const MulTab *tw, hipStream_t s);
+hipError_t launch_encode_kpow2(int kl, bool aligned, const uint8_t *msg, uint8_t *par, uint64_t L,
+                               uint32_t blocks, uint32_t msg_base, const MulTab *tw, hipStream_t s);
+hipError_t launch_recover_prep(RecPrep *out, uint32_t n, uint32_t e, const PosArg &pos,
+                               const uint8_t *gtab, hipStream_t s);
+hipError_t launch_recover(int nl, bool aligned, uint8_t *data, uint64_t L, const RecPrep *prep,
+                          const MulTab *tw, hipStream_t s);
+hipError_t launch_recompute(int tl, bool aligned, uint8_t *data, uint64_t L, const MulTab *tw,
+                            hipStream_t s);
+
+}  // namespace ars

@@ -1,0 +1,208 @@
+"""Benchmark: one (n=256, k=128) codeword with 1 MiB shards on one MI355X.
+
+A step is one encode (128 parity shards from 128 message shards) followed by
+one recover of 128 uniformly random erasures (seeded), both through the C ABI
+on device-resident buffers.  Throughput counts algorithmic bytes:
+encode n*L (k*L read + T*L written), recover (n+e)*L (n*L read + e*L written).
+
+Prints ONE JSON line (rank 0).  Usage:
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBPS = 8000.0
+METRIC = "encode/recover GB/s vs HBM roofline; batched p50/p99 latency vs plain calls"
+
+
+def load_pkg():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _pkg
+    return _pkg.load()
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(n, k, L, e, seconds):
+    """Oracle restatement of the reference's CPU path, one core, on the same
+    (n, k, L) codeword: encode + recover steps until `seconds` have passed."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    T = n - k
+    gfni = O.gfni_supported()
+    rng = np.random.default_rng(7)
+    msg = rng.integers(0, 256, k * L, dtype=np.uint8)
+    pos = np.sort(rng.permutation(n)[:e]).astype(np.uint8)
+    steps, t_enc, t_rec = 0, 0.0, 0.0
+    t_end = time.perf_counter() + seconds
+    while steps == 0 or time.perf_counter() < t_end:
+        par = np.zeros(T * L, np.uint8)
+        t0 = time.perf_counter()
+        O.encode(n, T, msg, L, parity=par, gfni=gfni)
+        t1 = time.perf_counter()
+        rx = np.concatenate([par, msg]).reshape(n, L)
+        rx[pos] = 0
+        rx = rx.reshape(-1)
+        t2 = time.perf_counter()
+        O.recover(n, T, rx, L, pos, gfni=gfni)
+        t3 = time.perf_counter()
+        t_enc += t1 - t0
+        t_rec += t3 - t2
+        steps += 1
+    byts = steps * (n * L + (n + e) * L)
+    return {
+        "value": round(byts / (t_enc + t_rec) / 1e9, 3),
+        "unit": "GB/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"{steps} encode+recover steps of the same (n={n},k={k}) codeword, L={L}, "
+                   f"e={e}; {'AVX-512BW+GFNI' if gfni else 'LUT'} restatement of "
+                   f"rs/src/kernel/gfni_kernel.rs + rs/src/codec.rs on {cpu_model()}"),
+        "encode_GBps": round(steps * n * L / t_enc / 1e9, 3),
+        "recover_GBps": round(steps * (n + e) * L / t_rec / 1e9, 3),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every figure is "
+              "per-GPU and the path shards by byte-column range with no exchange.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    ars = load_pkg()
+    n, k, L = 256, 128, 1 << 20
+    T = n - k
+    e = T
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(43)
+    cw = torch.empty(n * L, dtype=torch.uint8, device=dev)
+    cw[T * L:] = torch.randint(0, 256, (k * L,), dtype=torch.uint8, device=dev, generator=gen)
+    par, msg = cw[:T * L], cw[T * L:]
+    pos = np.sort(np.random.default_rng(43).permutation(n)[:e]).astype(np.uint8)
+    ctx = ars.default_context(0)
+    stream = torch.cuda.current_stream()
+
+    def encode():
+        ctx.encode(n, k, msg.data_ptr(), par.data_ptr(), L, 0, stream)
+
+    def recover():
+        ctx.recover(n, k, cw.data_ptr(), L, pos, stream)
+
+    # correctness gate (untimed): encode, erase, recover, compare
+    encode()
+    ref = cw.clone()
+    cwv = cw.view(n, L)
+    cwv[torch.from_numpy(pos.astype(np.int64)).to(dev)] = 0
+    recover()
+    torch.cuda.synchronize()
+    if not torch.equal(cw, ref):
+        print("bench: recovered codeword differs from the encoded one", file=sys.stderr)
+        sys.exit(1)
+
+    for _ in range(args.warmup):
+        encode()
+        recover()
+    torch.cuda.synchronize()
+
+    # whole step: encode + recover (per-call erasure plan included)
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t0.record(stream)
+    for _ in range(args.steps):
+        encode()
+        recover()
+    t1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms_step = t0.elapsed_time(t1) / args.steps
+    step_bytes = n * L + (n + e) * L
+    value = step_bytes / (ms_step * 1e-3) / 1e9
+
+    # per-kernel timing on the launch stream: encode kernel alone, and the
+    # fused recover kernel alone (erasure plan built once, outside)
+    plan = torch.empty(ars.lib().ars_recover_plan_size(), dtype=torch.uint8, device=dev)
+    ctx.recover_plan(n, k, pos, plan.data_ptr(), stream)
+
+    def timed(fn, reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        fn()
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps * 1e-3
+
+    reps = max(args.steps, 10)
+    s_enc = timed(encode, reps)
+    s_rec = timed(lambda: ctx.recover_with_plan(n, k, cw.data_ptr(), L, plan.data_ptr(), stream), reps)
+    enc_gbps = n * L / s_enc / 1e9
+    rec_gbps = (n + e) * L / s_rec / 1e9
+    if s_rec >= s_enc:
+        dom, dom_bytes, dom_s = "k_recover_tile<8,32>", (n + e) * L, s_rec
+    else:
+        dom, dom_bytes, dom_s = "k_encode_tile<7,64>", n * L, s_enc
+    achieved = dom_bytes / dom_s / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (uniform random bytes, seed 43; erasures: 128 seeded uniform positions)",
+        "config": {"workload": "single codeword (n=256,k=128), 1 MiB shards: encode 128 parity "
+                               "shards then recover 128 random erasures",
+                   "n": n, "k": k, "shard_len": L, "erasures": e, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": None},
+        "encode": {"GBps": round(enc_gbps, 1), "us": round(s_enc * 1e6, 2),
+                   "frac_hbm": round(enc_gbps / HBM_PEAK_GBPS, 4), "bytes": n * L},
+        "recover": {"GBps": round(rec_gbps, 1), "us": round(s_rec * 1e6, 2),
+                    "frac_hbm": round(rec_gbps / HBM_PEAK_GBPS, 4), "bytes": (n + e) * L},
+        "wall_s_timed_region": round(wall, 4),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, k, L, e, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,120 @@
+/*
+ * ars.h -- C ABI of the MI355X (gfx950) sharded additive-FFT Reed-Solomon
+ * codec.  Drop-in replacement for the sharded encode / erasure-recover path of
+ * the `rs` crate of vkomenda/additive-fft-reed-solomon.
+ *
+ * Code and layout are the reference's:
+ *   - GF(2^8) modulo 0x11d, evaluation points omega_0..omega_{n-1} spanned by
+ *     the Cantor basis, LCH novel-polynomial-basis FFT (rs/src/kernel/mod.rs:7-24);
+ *   - a codeword is n shards of shard_len bytes, shard i at [i*L, (i+1)*L);
+ *     parity shards 0..T-1 first, message shards T..n-1 after (T = n - k)
+ *     (rs/src/poly_11d_lut.rs:100-121, rs/benches/recover_erasure_shards.rs:100-101).
+ *
+ * (n, k) are run-time arguments (the reference takes N, T as const generics,
+ * rs/src/codec.rs:11).  n must be a power of two in [2, 256] and 1 <= k < n.
+ * The reference's contract also makes T = n - k a power of two
+ * (rs/src/lib.rs:16-19); codes whose T is not a power of two but whose k is
+ * (e.g. n = 256, k = 16) are accepted as an extension: they compute the same
+ * code (the unique systematic codeword with deg f < k) by a different route.
+ *
+ * All pointers named d_* are device pointers (HBM).  shard_len may be any
+ * value >= 1; shard_len % 4 == 0 with 4-byte aligned pointers takes the
+ * dword path.  No entry point allocates on the hot path except
+ * ars_recover (a small stream-ordered table, freed on the same stream) and
+ * the *_host variants (staging buffers).
+ *
+ * Return codes: >= 0 success (ars_recover: 1 = the reference's `true`,
+ * 0 = `false` because more than T erasures), < 0 = ARS_E_* below, for inputs
+ * the reference would panic on or silently mis-compute.
+ */
+#ifndef ARS_H
+#define ARS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ARS_OK 0
+#define ARS_TRUE 1
+#define ARS_FALSE 0
+#define ARS_E_INVAL (-1)       /* bad (n, k), shard_len == 0, or bad flags */
+#define ARS_E_POSITION (-2)    /* erasure position >= n */
+#define ARS_E_DUPLICATE (-3)   /* repeated erasure position */
+#define ARS_E_NULL (-4)        /* null pointer where data is required */
+#define ARS_E_HIP (-5)         /* HIP runtime error (message: ars_last_error) */
+#define ARS_E_NOMEM (-6)       /* device or host allocation failed */
+#define ARS_E_UNSUPPORTED (-7) /* neither T nor k is a power of two */
+#define ARS_E_BUDGET (-8)      /* serving: request exceeds the HBM budget */
+#define ARS_E_STATE (-9)       /* serving / batch object used out of order */
+
+/* Encode flags. */
+#define ARS_ENC_ACCUMULATE 1u  /* XOR the result into parity the way the
+                                  reference does (rs/src/codec.rs:259-264):
+                                  parity_out = FFT_T(parity_in ^ sum_i IFFT_T(chunk_i)).
+                                  Default: overwrite, equal to the reference
+                                  called with zeroed parity. */
+
+typedef struct ars_ctx ars_ctx;
+
+/* Create a context on HIP device `device`: uploads the twiddle tables and
+ * keeps them resident.  One context may be used from several host threads on
+ * different streams. */
+int ars_create(int device, ars_ctx **out);
+int ars_destroy(ars_ctx *ctx);
+const char *ars_last_error(void);
+const char *ars_version(void);
+
+/* Codec::encode_systematic_sharded (rs/src/codec.rs:241-265).
+ * d_message: k*L bytes (message shards T..n-1), d_parity: T*L bytes.
+ * The reference's `workspace` argument is not needed (tiles live in LDS).
+ * Asynchronous on `stream` (a hipStream_t; NULL = default stream). */
+int ars_encode(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *d_message,
+               uint8_t *d_parity, size_t shard_len, uint32_t flags, void *stream);
+
+/* Codec::recover_erasure_shards (rs/src/codec.rs:479-569).
+ * d_received: n*L bytes, in place.  positions: HOST array of e shard indices
+ * (any order).  Writes every erased shard p; as in the reference, whatever p
+ * held before is XORed into the result (zero-filled erasures come back as
+ * the original shard).  Returns 1, 0 if e > n - k (received untouched), or
+ * ARS_E_*.  Asynchronous on `stream`. */
+int ars_recover(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_t shard_len,
+                const uint8_t *positions, uint32_t e, void *stream);
+
+/* Two-step recover for callers that reuse one erasure pattern or want the
+ * per-pattern work off the critical path (serving, batching, timing the
+ * main kernel alone).  ars_recover_plan validates the pattern exactly like
+ * ars_recover and writes, on `stream`, a device-resident plan of
+ * ars_recover_plan_size() bytes to d_plan: per shard, the multiplier
+ * Pi(omega_i) = prod_{p in E} (omega_i + omega_p) or, for erased shards,
+ * 1 / Pi'(omega_i) -- the reference's erasure locator / denominators
+ * (rs/src/codec.rs:436-465, 527-536) in evaluated form.
+ * ars_recover_with_plan then runs the fused recover for that pattern; the
+ * plan must come from ars_recover_plan with the same (n, k).  Return codes
+ * as for ars_recover (the plan call returns 0 for e > T and 1 otherwise;
+ * with_plan returns 1). */
+size_t ars_recover_plan_size(void);
+int ars_recover_plan(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *positions, uint32_t e,
+                     void *d_plan, void *stream);
+int ars_recover_with_plan(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received,
+                          size_t shard_len, const void *d_plan, void *stream);
+
+/* Codec::recompute_data_from_parity_sharded (rs/src/codec.rs:321-340):
+ * n = 2T only; rewrites the message shards T..n-1 from the parity shards. */
+int ars_recompute_data(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received,
+                       size_t shard_len, void *stream);
+
+/* Host-slice variants: same semantics on host buffers, synchronous; they
+ * stage through device memory (PCIe-bound, not the HBM roofline path). */
+int ars_encode_host(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *message,
+                    uint8_t *parity, size_t shard_len, uint32_t flags);
+int ars_recover_host(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *received, size_t shard_len,
+                     const uint8_t *positions, uint32_t e);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ARS_H */

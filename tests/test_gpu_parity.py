@@ -1,0 +1,207 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle, byte
+for byte, on seeded inputs; plus size-independent properties at full size."""
+import numpy as np
+import pytest
+
+import _pkg
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def ars():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return _pkg.load()
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def gpu_encode(ars, n, k, msg, L, parity_in=None, accumulate=False):
+    c = ars.Codec(n, k)
+    dm = dev(msg)
+    dp = dev(parity_in) if parity_in is not None else torch.zeros((n - k) * L, dtype=torch.uint8, device="cuda")
+    c.encode_systematic_sharded(dm, dp, None, L, accumulate=accumulate)
+    return host(dp)
+
+
+def gpu_recover(ars, n, k, rx, L, pos):
+    c = ars.Codec(n, k)
+    d = dev(rx)
+    ok = c.recover_erasure_shards(d, None, L, pos)
+    return ok, host(d)
+
+
+POW2 = [(n, T) for n in (2, 4, 8, 16, 32, 64, 128, 256) for T in (1, 2, 4, 8, 16, 32, 64, 128) if T < n]
+
+
+@pytest.mark.parametrize("n,T", POW2)
+@pytest.mark.parametrize("L", [1, 64, 100, 1030])
+def test_encode_matches_oracle(ars, n, T, L):
+    rng = np.random.default_rng(n * 7919 + T * 31 + L)
+    k = n - T
+    msg = rng.integers(0, 256, k * L, dtype=np.uint8)
+    exp = O.encode(n, T, msg, L)
+    got = gpu_encode(ars, n, k, msg, L)
+    assert np.array_equal(got, exp), (n, T, L)
+
+
+@pytest.mark.parametrize("n,T", [(4, 2), (32, 8), (64, 32), (256, 128), (256, 32)])
+def test_encode_accumulate_matches_reference_semantics(ars, n, T):
+    rng = np.random.default_rng(17 + n + T)
+    L = 256
+    k = n - T
+    msg = rng.integers(0, 256, k * L, dtype=np.uint8)
+    pin = rng.integers(0, 256, T * L, dtype=np.uint8)
+    exp = O.encode(n, T, msg, L, parity=pin.copy())
+    got = gpu_encode(ars, n, k, msg, L, parity_in=pin, accumulate=True)
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("n,k", [(8, 2), (16, 4), (32, 8), (64, 16), (256, 16), (256, 32), (256, 64)])
+def test_encode_kpow2_extension(ars, n, k):
+    rng = np.random.default_rng(n * 3 + k)
+    L = 128
+    msg = rng.integers(0, 256, k * L, dtype=np.uint8)
+    exp = O.ext_encode_kpow2(n, k, msg, L)
+    assert np.array_equal(gpu_encode(ars, n, k, msg, L), exp)
+
+
+def make_cw(n, k, L, rng):
+    msg = rng.integers(0, 256, k * L, dtype=np.uint8)
+    return np.concatenate([O.encode_any(n, k, msg, L), msg])
+
+
+@pytest.mark.parametrize("n,T", POW2)
+@pytest.mark.parametrize("L", [1, 64, 132])
+def test_recover_matches_oracle(ars, n, T, L):
+    rng = np.random.default_rng(n * 131 + T * 17 + L)
+    k = n - T
+    cw = make_cw(n, k, L, rng)
+    for e in sorted({1, max(1, T // 2), T}):
+        pos = rng.permutation(n)[:e].astype(np.uint8)
+        rx = cw.copy().reshape(n, L)
+        rx[pos] = rng.integers(0, 256, (e, L), dtype=np.uint8) if e % 2 else 0
+        rx = rx.reshape(-1)
+        exp = rx.copy()
+        assert O.recover(n, T, exp, L, pos) == 1
+        ok, got = gpu_recover(ars, n, k, rx, L, pos)
+        assert ok
+        assert np.array_equal(got, exp), (n, T, L, e)
+
+
+@pytest.mark.parametrize("n,T", POW2)
+def test_recover_reference_odd_pattern(ars, n, T):
+    # rs/src/poly_11d_lut.rs:536-566 at L = 1 and L = 64
+    k = n - T
+    for L in (1, 64):
+        msg = np.array([(T + i + j) & 0xFF for i in range(k) for j in range(L)], np.uint8)
+        cw = np.concatenate([O.encode(n, T, msg, L), msg])
+        pos = list(range(1, 2 * T, 2))
+        rx = cw.copy().reshape(n, L)
+        rx[pos] = 0
+        ok, got = gpu_recover(ars, n, k, rx.reshape(-1), L, pos)
+        assert ok and np.array_equal(got, cw)
+
+
+@pytest.mark.parametrize("n,k", [(16, 4), (32, 8), (256, 16)])
+def test_recover_kpow2_extension(ars, n, k):
+    rng = np.random.default_rng(5 * n + k)
+    L = 64
+    T = n - k
+    cw = make_cw(n, k, L, rng)
+    for e in (1, T // 3, T):
+        pos = rng.permutation(n)[:e].astype(np.uint8)
+        rx = cw.copy().reshape(n, L)
+        rx[pos] = 0
+        ok, got = gpu_recover(ars, n, k, rx.reshape(-1), L, pos)
+        assert ok and np.array_equal(got, cw), (n, k, e)
+
+
+def test_recover_edge_cases(ars):
+    rng = np.random.default_rng(1)
+    n, k, L = 64, 32, 64
+    cw = make_cw(n, k, L, rng)
+    # e > T -> false, untouched (rs/src/codec.rs:491-493)
+    ok, got = gpu_recover(ars, n, k, cw.copy(), L, list(range(33)))
+    assert not ok and np.array_equal(got, cw)
+    # e == 0 -> true, untouched
+    ok, got = gpu_recover(ars, n, k, cw.copy(), L, [])
+    assert ok and np.array_equal(got, cw)
+    c = ars.Codec(n, k)
+    d = dev(cw)
+    with pytest.raises(ars.ArsError) as ei:
+        c.recover_erasure_shards(d, None, L, [3, 64])
+    assert ei.value.code == ars.ARS_E_POSITION
+    with pytest.raises(ars.ArsError) as ei:
+        c.recover_erasure_shards(d, None, L, [3, 5, 3])
+    assert ei.value.code == ars.ARS_E_DUPLICATE
+    # bad (n, k) is rejected at the C ABI (the reference's const generics
+    # would not compile; rs/src/lib.rs:16-19)
+    for bad_n, bad_k in [(48, 16), (512, 256), (64, 64), (64, 0), (256, 100)]:
+        with pytest.raises(ars.ArsError) as ei:
+            c.ctx.encode(bad_n, bad_k, d.data_ptr(), d.data_ptr(), L)
+        assert ei.value.code in (ars.ARS_E_INVAL, ars.ARS_E_UNSUPPORTED)
+
+
+def test_recompute_data_matches_oracle(ars):
+    rng = np.random.default_rng(2)
+    for n in (2, 4, 8, 16, 32, 64, 128, 256):
+        T, L = n // 2, 96
+        cw = make_cw(n, n - T, L, rng)
+        rx = cw.copy()
+        rx[T * L:] = 0
+        d = dev(rx)
+        ars.Codec(n, n - T).recompute_data_from_parity_sharded(d, None, L)
+        assert np.array_equal(host(d), cw), n
+
+
+def test_host_slice_variants(ars):
+    rng = np.random.default_rng(3)
+    n, k, L = 128, 96, 256
+    msg = rng.integers(0, 256, k * L, dtype=np.uint8)
+    par = np.zeros((n - k) * L, np.uint8)
+    ars.Codec(n, k).encode_systematic_sharded(msg, par, None, L)
+    assert np.array_equal(par, O.encode(n, n - k, msg, L))
+    cw = np.concatenate([par, msg])
+    pos = rng.permutation(n)[:32].astype(np.uint8)
+    rx = cw.copy().reshape(n, L)
+    rx[pos] = 0
+    rx = rx.reshape(-1).copy()
+    assert ars.Codec(n, k).recover_erasure_shards(rx, None, L, pos)
+    assert np.array_equal(rx, cw)
+
+
+def test_full_size_256_128_1mib(ars):
+    """The headline shape: (256,128), 1 MiB shards.  Encode is compared with
+    the oracle byte for byte; recover must return the original codeword
+    (uniqueness) and equal the oracle's recover."""
+    n, k, L = 256, 128, 1 << 20
+    T = n - k
+    rng = np.random.default_rng(43)
+    msg = rng.integers(0, 256, k * L, dtype=np.uint8)
+    use_gfni = O.gfni_supported()
+    exp_par = O.encode(n, T, msg, L, gfni=use_gfni)
+    dm = dev(msg)
+    dp = torch.zeros(T * L, dtype=torch.uint8, device="cuda")
+    ars.Codec(n, k).encode_systematic_sharded(dm, dp, None, L, accumulate=False)
+    got_par = host(dp)
+    assert np.array_equal(got_par, exp_par)
+    cw = torch.cat([dp, dm])
+    pos = rng.permutation(n)[:T].astype(np.uint8)
+    cwv = cw.view(n, L)
+    cwv[torch.from_numpy(pos.astype(np.int64)).cuda()] = 0
+    assert ars.Codec(n, k).recover_erasure_shards(cw, None, L, pos)
+    got = host(cw)
+    assert np.array_equal(got[:T * L], exp_par)
+    assert np.array_equal(got[T * L:], msg)
