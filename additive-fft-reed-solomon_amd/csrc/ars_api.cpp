@@ -44,6 +44,8 @@ int check_code(uint32_t n, uint32_t k, size_t L) {
     if (n < 2 || n > 256 || !is_pow2(n)) return fail(ARS_E_INVAL, "n must be a power of two in [2, 256]");
     if (k < 1 || k >= n) return fail(ARS_E_INVAL, "k must satisfy 1 <= k < n");
     if (L == 0) return fail(ARS_E_INVAL, "shard_len must be >= 1");
+    if ((uint64_t)n * L > 0xFFFFFFFFull)
+        return fail(ARS_E_INVAL, "codeword larger than 4 GiB (n * shard_len must fit 32 bits)");
     const uint32_t T = n - k;
     if (!is_pow2(T) && !is_pow2(k))
         return fail(ARS_E_UNSUPPORTED, "neither T = n - k nor k is a power of two");

@@ -54,24 +54,32 @@ __device__ __forceinline__ MulTab lds_tab(const MulTab *t, uint32_t i) {
 }
 
 // One butterfly level over R = 2^RL registers.  Register j holds the point
-// with absolute index  base + (j << SH)  (base has zero bits in
-// [SH, SH + RL)); LV is the absolute level.  The twiddle of the block whose
-// first point has absolute index o is s_LV(omega_o) (LNH Alg. 1/2 unrolled:
-// the recursion of rs/src/kernel/lut_kernel.rs:28-82 passes beta + v_{k-1}
-// to the upper half, so a block at offset o sees beta + omega_o).
+// with absolute index  hb + (j << SH) + c,  where hb has zero bits below
+// SH + RL and c < 2^SH never reaches a twiddle (LV >= SH).  The twiddle of
+// the block whose first point has absolute index o is s_LV(omega_o): the
+// recursion of rs/src/kernel/lut_kernel.rs:28-82 hands beta + v_{k-1} to the
+// upper half, so a block at offset o sees beta + omega_o, and beta is itself
+// omega_B for the shifts the codec uses (rs/src/codec.rs:257, 515).
 // FWD: a ^= t*b; b ^= a   (rs/src/kernel/lut_kernel.rs:11-18)
 // INV: b ^= a; a ^= t*b   (rs/src/kernel/lut_kernel.rs:20-26)
-template <int RL, int SH, int LV, bool INV>
-__device__ __forceinline__ void level_pass(uint32_t *x, uint32_t base, const MulTab *tw) {
+// BZ = hb is known to be zero: blocks at o = 0 have twiddle s_LV(0) = 0 and
+// become XOR-only at compile time.  Otherwise every block multiplies (the
+// zero twiddle's table is all-zero, so a lane whose block starts at 0 still
+// computes the right value) and no lane-divergent branch is emitted.
+template <int RL, int SH, int LV, bool INV, bool BZ>
+__device__ __forceinline__ void level_pass(uint32_t *x, uint32_t hb, const MulTab *tw) {
     constexpr int lb = LV - SH;
     static_assert(lb >= 0 && lb < RL, "level outside the register range");
     constexpr int half = 1 << lb;
+    const uint32_t tbase = (uint32_t)lvl_off(LV) + (BZ ? 0u : (hb >> (LV + 1)));
 #pragma unroll
     for (int blk = 0; blk < (1 << RL); blk += 2 * half) {
-        const uint32_t o = (base | ((uint32_t)blk << SH)) & ~((2u << LV) - 1u);
-        const uint32_t ti = o >> (LV + 1);
-        if (ti != 0u) {
-            const MulTab m = lds_tab(tw, (uint32_t)lvl_off(LV) + ti);
+        const uint32_t cpart = ((uint32_t)blk << SH) >> (LV + 1);
+        if (BZ && cpart == 0u) {
+#pragma unroll
+            for (int i = 0; i < half; ++i) x[blk + i + half] ^= x[blk + i];
+        } else {
+            const MulTab m = lds_tab(tw, tbase + cpart);
 #pragma unroll
             for (int i = 0; i < half; ++i) {
                 uint32_t &a = x[blk + i];
@@ -84,50 +92,66 @@ __device__ __forceinline__ void level_pass(uint32_t *x, uint32_t base, const Mul
                     b ^= a;
                 }
             }
-        } else {
-#pragma unroll
-            for (int i = 0; i < half; ++i) x[blk + i + half] ^= x[blk + i];
         }
     }
 }
 
 // Levels [L0, L1): ascending inverse butterflies (IFFT) or descending
 // forward butterflies (FFT).
-template <int RL, int SH, int L0, int L1, bool INV>
-__device__ __forceinline__ void run_levels(uint32_t *x, uint32_t base, const MulTab *tw) {
+template <int RL, int SH, int L0, int L1, bool INV, bool BZ>
+__device__ __forceinline__ void run_levels(uint32_t *x, uint32_t hb, const MulTab *tw) {
     if constexpr (L0 < L1) {
         if constexpr (INV) {
-            level_pass<RL, SH, L0, true>(x, base, tw);
-            run_levels<RL, SH, L0 + 1, L1, true>(x, base, tw);
+            level_pass<RL, SH, L0, true, BZ>(x, hb, tw);
+            run_levels<RL, SH, L0 + 1, L1, true, BZ>(x, hb, tw);
         } else {
-            level_pass<RL, SH, L1 - 1, false>(x, base, tw);
-            run_levels<RL, SH, L0, L1 - 1, false>(x, base, tw);
+            level_pass<RL, SH, L1 - 1, false, BZ>(x, hb, tw);
+            run_levels<RL, SH, L0, L1 - 1, false, BZ>(x, hb, tw);
         }
     }
 }
 
-// Load / store one column quad of one shard.  nb = valid bytes (0..4).
+// Shard rows of one buffer, addressed as (per-lane byte offset, uniform
+// shard byte offset) through a buffer resource, so the per-shard offset is
+// scalar.  AL: shard_len % 4 == 0 and a 4-byte aligned base, so a column quad
+// is one dword load; lanes whose quad lies past the end of the shard read a
+// clamped in-bounds quad (byte columns are independent, so the garbage never
+// reaches a stored column) and store nothing.  !AL: byte loads and stores
+// predicated on the nb valid bytes of the quad.
+struct Rows {
+    __amdgpu_buffer_rsrc_t r;
+};
+
+__device__ __forceinline__ Rows make_rows(const void *base, uint32_t bytes) {
+    Rows w;
+    w.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+    return w;
+}
+
 template <bool AL>
-__device__ __forceinline__ uint32_t ldq(const uint8_t *__restrict__ p, uint32_t nb) {
+__device__ __forceinline__ uint32_t ldq(const Rows &w, uint32_t voff, uint32_t soff, uint32_t nb) {
     if constexpr (AL) {
-        return nb ? *reinterpret_cast<const uint32_t *>(p) : 0u;
+        (void)nb;
+        return __builtin_amdgcn_raw_buffer_load_b32(w.r, (int)voff, (int)soff, 0);
     } else {
         uint32_t v = 0;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            if ((uint32_t)i < nb) v |= (uint32_t)p[i] << (8 * i);
+            if ((uint32_t)i < nb)
+                v |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(w.r, (int)(voff + i), (int)soff, 0) << (8 * i);
         return v;
     }
 }
 
 template <bool AL>
-__device__ __forceinline__ void stq(uint8_t *__restrict__ p, uint32_t v, uint32_t nb) {
+__device__ __forceinline__ void stq(const Rows &w, uint32_t voff, uint32_t soff, uint32_t v, uint32_t nb) {
     if constexpr (AL) {
-        if (nb) *reinterpret_cast<uint32_t *>(p) = v;
+        if (nb) __builtin_amdgcn_raw_buffer_store_b32(v, w.r, (int)voff, (int)soff, 0);
     } else {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-            if ((uint32_t)i < nb) p[i] = (uint8_t)(v >> (8 * i));
+            if ((uint32_t)i < nb)
+                __builtin_amdgcn_raw_buffer_store_b8((uint8_t)(v >> (8 * i)), w.r, (int)(voff + i), (int)soff, 0);
     }
 }
 

@@ -22,14 +22,14 @@ struct PosArg {
 constexpr int kGtabBytes = 1024;
 
 hipError_t launch_encode_ref(int tl, bool aligned, bool accumulate, const uint8_t *msg, uint8_t *par,
-                             uint64_t L, uint32_t chunks, const MulTab *tw, hipStream_t s);
-hipError_t launch_encode_kpow2(int kl, bool aligned, const uint8_t *msg, uint8_t *par, uint64_t L,
+                             uint32_t L, uint32_t chunks, const MulTab *tw, hipStream_t s);
+hipError_t launch_encode_kpow2(int kl, bool aligned, const uint8_t *msg, uint8_t *par, uint32_t L,
                                uint32_t blocks, uint32_t msg_base, const MulTab *tw, hipStream_t s);
 hipError_t launch_recover_prep(RecPrep *out, uint32_t n, uint32_t e, const PosArg &pos,
                                const uint8_t *gtab, hipStream_t s);
-hipError_t launch_recover(int nl, bool aligned, uint8_t *data, uint64_t L, const RecPrep *prep,
+hipError_t launch_recover(int nl, bool aligned, uint8_t *data, uint32_t L, const RecPrep *prep,
                           const MulTab *tw, hipStream_t s);
-hipError_t launch_recompute(int tl, bool aligned, uint8_t *data, uint64_t L, const MulTab *tw,
+hipError_t launch_recompute(int tl, bool aligned, uint8_t *data, uint32_t L, const MulTab *tw,
                             hipStream_t s);
 
 }  // namespace ars

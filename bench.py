@@ -166,6 +166,13 @@ def main():
     reps = max(args.steps, 10)
     s_enc = timed(encode, reps)
     s_rec = timed(lambda: ctx.recover_with_plan(n, k, cw.data_ptr(), L, plan.data_ptr(), stream), reps)
+    # attainable HBM bandwidth: a plain device-to-device copy of the same
+    # size as the recover's input (read + write counted)
+    src = cw
+    dst = torch.empty_like(cw)
+    s_copy = timed(lambda: dst.copy_(src), reps)
+    copy_gbps = 2 * n * L / s_copy / 1e9
+    del dst
     enc_gbps = n * L / s_enc / 1e9
     rec_gbps = (n + e) * L / s_rec / 1e9
     if s_rec >= s_enc:
@@ -197,6 +204,8 @@ def main():
                    "frac_hbm": round(enc_gbps / HBM_PEAK_GBPS, 4), "bytes": n * L},
         "recover": {"GBps": round(rec_gbps, 1), "us": round(s_rec * 1e6, 2),
                     "frac_hbm": round(rec_gbps / HBM_PEAK_GBPS, 4), "bytes": (n + e) * L},
+        "copy_reference": {"GBps": round(copy_gbps, 1), "bytes_read_plus_written": 2 * n * L,
+                           "frac_hbm": round(copy_gbps / HBM_PEAK_GBPS, 4)},
         "wall_s_timed_region": round(wall, 4),
     }
     if not args.no_cpu:

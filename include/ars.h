@@ -18,8 +18,8 @@
  * code (the unique systematic codeword with deg f < k) by a different route.
  *
  * All pointers named d_* are device pointers (HBM).  shard_len may be any
- * value >= 1; shard_len % 4 == 0 with 4-byte aligned pointers takes the
- * dword path.  No entry point allocates on the hot path except
+ * value >= 1 with n * shard_len < 4 GiB; shard_len % 4 == 0 with 4-byte
+ * aligned pointers takes the dword path.  No entry point allocates on the hot path except
  * ars_recover (a small stream-ordered table, freed on the same stream) and
  * the *_host variants (staging buffers).
  *
