@@ -77,6 +77,18 @@ def cpu_baseline(n, k, L, e, seconds):
     }
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (profiles/traffic_256x128_1MiB.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_256x128_1MiB.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)["kernels"].get(kernel)
+        return rec["traffic_bytes"] if rec else None
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,10 +188,11 @@ def main():
     enc_gbps = n * L / s_enc / 1e9
     rec_gbps = (n + e) * L / s_rec / 1e9
     if s_rec >= s_enc:
-        dom, dom_bytes, dom_s = "k_recover_tile<8,32>", (n + e) * L, s_rec
+        dom, dom_bytes, dom_s = "k_recover_tile<8, 32, true, 1>", (n + e) * L, s_rec
     else:
-        dom, dom_bytes, dom_s = "k_encode_tile<7,64>", n * L, s_enc
+        dom, dom_bytes, dom_s = "k_encode_tile<7, 64, true, false>", n * L, s_enc
     achieved = dom_bytes / dom_s / 1e9
+    traffic = measured_traffic(dom)
 
     out = {
         "metric": METRIC,
@@ -199,7 +212,7 @@ def main():
                    "n": n, "k": k, "shard_len": L, "erasures": e, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": None},
+                     "traffic": traffic, "algorithmic_bytes": dom_bytes},
         "encode": {"GBps": round(enc_gbps, 1), "us": round(s_enc * 1e6, 2),
                    "frac_hbm": round(enc_gbps / HBM_PEAK_GBPS, 4), "bytes": n * L},
         "recover": {"GBps": round(rec_gbps, 1), "us": round(s_rec * 1e6, 2),
