@@ -57,7 +57,33 @@ SIGNATURES = {
     "ars_recover_with_plan": (_i, [_vp, _u32, _u32, _vp, _sz, _vp, _vp]),
     "ars_encode_host": (_i, [_vp, _u32, _u32, _vp, _vp, _sz, _u32]),
     "ars_recover_host": (_i, [_vp, _u32, _u32, _vp, _sz, _vp, _u32]),
+    "ars_batch_create": (_i, [_vp, _u32, ctypes.POINTER(_vp)]),
+    "ars_batch_destroy": (_i, [_vp]),
+    "ars_batch_run": (_i, [_vp, _vp, _u32, _u32, _vp]),
+    "ars_batch_info_get": (_i, [_vp, _vp]),
+    "ars_batch_set_blocks_per_cu": (_i, [_vp, _u32]),
+    "ars_run_jobs_plain": (_i, [_vp, _vp, _u32, _vp]),
 }
+
+ARS_OP_ENCODE = 1
+ARS_OP_RECOVER = 2
+ARS_BATCH_NO_GRAPH = 1
+
+
+class Job(ctypes.Structure):
+    """struct ars_job (include/ars.h)."""
+    _fields_ = [("op", ctypes.c_uint32), ("n", ctypes.c_uint32), ("k", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32), ("shard_len", ctypes.c_uint64),
+                ("d_message", ctypes.c_void_p), ("d_parity", ctypes.c_void_p),
+                ("d_codeword", ctypes.c_void_p), ("positions", ctypes.c_void_p),
+                ("e", ctypes.c_uint32), ("status", ctypes.c_int32)]
+
+
+class BatchInfo(ctypes.Structure):
+    """struct ars_batch_info (include/ars.h)."""
+    _fields_ = [("jobs", ctypes.c_uint32), ("items", ctypes.c_uint32), ("segments", ctypes.c_uint32),
+                ("plans", ctypes.c_uint32), ("fallback_jobs", ctypes.c_uint32), ("grid", ctypes.c_uint32),
+                ("algorithmic_bytes", ctypes.c_uint64)]
 
 _lib = None
 
@@ -156,6 +182,74 @@ class Context:
     def recompute_data(self, n, k, d_received, shard_len, stream=None):
         return _check(lib().ars_recompute_data(self.handle, n, k, d_received, shard_len,
                                                _stream_ptr(stream)))
+
+
+class Batch:
+    """Many encode / recover jobs in one persistent launch (graph replay).
+
+    run() takes a list of job dicts:
+      {"op": "encode", "n", "k", "shard_len", "message": dev_tensor, "parity": dev_tensor,
+       "accumulate": bool}
+      {"op": "recover", "n", "k", "shard_len", "codeword": dev_tensor, "positions": seq}
+    and returns the per-job statuses (as ars_encode / ars_recover would)."""
+
+    def __init__(self, max_jobs, ctx=None):
+        self.ctx = ctx or default_context(0)
+        h = ctypes.c_void_p()
+        _check(lib().ars_batch_create(self.ctx.handle, max_jobs, ctypes.byref(h)))
+        self.handle = h
+        self.max_jobs = max_jobs
+
+    def close(self):
+        if self.handle:
+            lib().ars_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_blocks_per_cu(self, bpc):
+        _check(lib().ars_batch_set_blocks_per_cu(self.handle, bpc))
+
+    def info(self):
+        inf = BatchInfo()
+        _check(lib().ars_batch_info_get(self.handle, ctypes.byref(inf)))
+        return {f: getattr(inf, f) for f, _ in BatchInfo._fields_}
+
+    @staticmethod
+    def make_jobs(specs):
+        arr = (Job * len(specs))()
+        keep = []
+        for i, sp in enumerate(specs):
+            j = arr[i]
+            j.n, j.k, j.shard_len = sp["n"], sp["k"], sp["shard_len"]
+            if sp["op"] == "encode":
+                j.op = ARS_OP_ENCODE
+                j.flags = ARS_ENC_ACCUMULATE if sp.get("accumulate") else 0
+                j.d_message = _ptr(sp["message"])[0]
+                j.d_parity = _ptr(sp["parity"])[0]
+            else:
+                j.op = ARS_OP_RECOVER
+                pos = np.ascontiguousarray(np.asarray(sp["positions"], dtype=np.uint8))
+                keep.append(pos)
+                j.d_codeword = _ptr(sp["codeword"])[0]
+                j.positions = pos.ctypes.data if len(pos) else None
+                j.e = len(pos)
+        return arr, keep
+
+    def run(self, specs, stream=None, graph=True):
+        arr, keep = self.make_jobs(specs)
+        _check(lib().ars_batch_run(self.handle, arr, len(specs), 0 if graph else ARS_BATCH_NO_GRAPH,
+                                   _stream_ptr(stream)))
+        return [arr[i].status for i in range(len(specs))]
+
+    def run_jobs(self, arr, count, stream=None, graph=True):
+        """Run a prepared ctypes Job array (no per-call Python conversion)."""
+        return _check(lib().ars_batch_run(self.handle, arr, count, 0 if graph else ARS_BATCH_NO_GRAPH,
+                                          _stream_ptr(stream)))
 
 
 _default_ctx = {}

@@ -10,25 +10,20 @@
 #include <string>
 
 #include "../../include/ars.h"
+#include "ars_ctx.hpp"
 #include "ars_gf.hpp"
 #include "ars_kernels.hpp"
 
-struct ars_ctx {
-    int device = 0;
-    ars::MulTab *d_tw = nullptr;  // 255 twiddle multipliers, resident
-    uint8_t *d_gtab = nullptr;    // exp / log / omega for the recover prep kernel
-};
-
-namespace {
+namespace ars {
 
 thread_local std::string g_err;
 
-int fail(int code, const char *msg) {
+int set_error(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
 
-int hip_fail(hipError_t e, const char *where) {
+int hip_error(hipError_t e, const char *where) {
     g_err = std::string(where) + ": " + hipGetErrorString(e);
     return ARS_E_HIP;
 }
@@ -40,7 +35,20 @@ int ilog2(uint32_t v) {
     return r;
 }
 
-int check_code(uint32_t n, uint32_t k, size_t L) {
+}  // namespace ars
+
+namespace {
+
+using ars::g_err;
+using ars::ilog2;
+using ars::is_pow2;
+
+int fail(int code, const char *msg) { return ars::set_error(code, msg); }
+int hip_fail(hipError_t e, const char *where) { return ars::hip_error(e, where); }
+
+}  // namespace
+
+int ars::check_code(uint32_t n, uint32_t k, size_t L) {
     if (n < 2 || n > 256 || !is_pow2(n)) return fail(ARS_E_INVAL, "n must be a power of two in [2, 256]");
     if (k < 1 || k >= n) return fail(ARS_E_INVAL, "k must satisfy 1 <= k < n");
     if (L == 0) return fail(ARS_E_INVAL, "shard_len must be >= 1");
@@ -52,8 +60,9 @@ int check_code(uint32_t n, uint32_t k, size_t L) {
     return ARS_OK;
 }
 
+namespace {
+using ars::check_code;
 bool aligned4(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; }
-
 }  // namespace
 
 extern "C" {
@@ -69,6 +78,8 @@ int ars_create(int device, ars_ctx **out) {
     ars_ctx *c = new (std::nothrow) ars_ctx;
     if (!c) return fail(ARS_E_NOMEM, "context allocation failed");
     c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
     const ars::GfTables &g = ars::gf();
     ars::MulTab tw[ars::kTwiddleEntries];
     for (int l = 0; l < 8; ++l)

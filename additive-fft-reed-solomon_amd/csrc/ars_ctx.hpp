@@ -1,0 +1,23 @@
+// ars_ctx.hpp -- the per-device context shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "ars_gf.hpp"
+
+struct ars_ctx {
+    int device = 0;
+    ars::MulTab *d_tw = nullptr;  // 255 twiddle multipliers, resident
+    uint8_t *d_gtab = nullptr;    // exp / log / omega for the recover plan kernels
+    int num_cus = 0;
+};
+
+namespace ars {
+int set_error(int code, const std::string &msg);
+int hip_error(hipError_t e, const char *where);
+int check_code(uint32_t n, uint32_t k, size_t L);
+bool is_pow2(uint32_t v);
+int ilog2(uint32_t v);
+}  // namespace ars

@@ -1,0 +1,400 @@
+// ars_tiles.hpp -- the fused encode / recover / recompute tile bodies, shared
+// by the one-codeword-per-launch kernels (ars_kernels.hip) and the batched
+// persistent kernel (ars_batch.hip).
+//
+// Work decomposition.  A workgroup owns a tile of PQ column quads (4*PQ byte
+// columns) of every shard of one transform.  For a transform of 2^TL points:
+//   TL <= 4 : each lane keeps all 2^TL shards of its quad in registers and
+//             runs every butterfly level there (no LDS traffic).
+//   TL >= 5 : Q = 2^(TL-4) lanes share a quad.  Phase A: lane q holds the 16
+//             consecutive shards q*16..q*16+15 and runs levels 0..3 in
+//             registers.  One LDS exchange regroups the tile so lane q holds
+//             shards {h*16 + c : c in its set of 16/Q}; phase B runs levels
+//             4..TL-1 in registers.  The FFT walks the same two phases in the
+//             opposite order, so a transform costs one LDS exchange.
+// Point indices are absolute (they include the transform's shift beta =
+// omega_B), so every transform of every code reads the same 255-entry
+// twiddle table (see level_pass).
+//
+// The bodies are templated on an I/O policy:
+//   io.ld_in(g, j)   quad of input row g + j   (g lane-varying, j compile-time)
+//   io.ld_out(g, j)  quad of output row g + j  (encode accumulate only)
+//   io.st_out(g, j, v)
+// and, for recover, on a per-shard multiplier source:
+//   tabs.tab(s) -> MulTab, tabs.erased(s) -> bool.
+// Columns are independent, so lanes may even belong to different codewords
+// as long as they share the code shape (the batched kernel uses that).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "ars_device.hpp"
+#include "ars_kernels.hpp"
+
+namespace ars {
+
+template <int TL, int PQ>
+struct Tile {
+    static constexpr int N = 1 << TL;
+    static constexpr int Q = N / 16;
+    static constexpr int RB = TL - 4;
+    static constexpr int GB = 16 / Q;
+    static constexpr int THREADS = Q * PQ;
+    static constexpr int WORDS = N * PQ;
+
+    __device__ static __forceinline__ void put_a(uint32_t *tb, const uint32_t (&x)[16], int q, int p) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) tb[(q * 16 + j) * PQ + p] = x[j];
+    }
+    __device__ static __forceinline__ void get_a(const uint32_t *tb, uint32_t (&x)[16], int q, int p) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) x[j] = tb[(q * 16 + j) * PQ + p];
+    }
+    __device__ static __forceinline__ void put_b(uint32_t *tb, const uint32_t (&y)[GB][Q], int q, int p) {
+#pragma unroll
+        for (int m = 0; m < GB; ++m)
+#pragma unroll
+            for (int h = 0; h < Q; ++h) tb[(h * 16 + q * GB + m) * PQ + p] = y[m][h];
+    }
+    __device__ static __forceinline__ void get_b(const uint32_t *tb, uint32_t (&y)[GB][Q], int q, int p) {
+#pragma unroll
+        for (int m = 0; m < GB; ++m)
+#pragma unroll
+            for (int h = 0; h < Q; ++h) y[m][h] = tb[(h * 16 + q * GB + m) * PQ + p];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Encode, reference algorithm (rs/src/codec.rs:241-265): for each message
+// chunk c, acc ^= IFFT_T(chunk c, omega_{(c+1)T}); parity = FFT_T(acc, 0).
+// Input rows: message shards (chunk c row j = c*T + j); output: parity.
+template <int TL, bool ACC, class IO>
+__device__ __forceinline__ void body_encode_small(IO &io, uint32_t chunks, const MulTab *tw) {
+    constexpr int T = 1 << TL;
+    uint32_t acc[T], x[T];
+#pragma unroll
+    for (int j = 0; j < T; ++j) acc[j] = ACC ? io.ld_out(0u, j) : 0u;
+    for (uint32_t c = 0; c < chunks; ++c) {
+#pragma unroll
+        for (int j = 0; j < T; ++j) x[j] = io.ld_in(c * T, j);
+        run_levels<TL, 0, 0, TL, true, false>(x, (c + 1) * T, tw);
+#pragma unroll
+        for (int j = 0; j < T; ++j) acc[j] ^= x[j];
+    }
+    run_levels<TL, 0, 0, TL, false, true>(acc, 0u, tw);
+#pragma unroll
+    for (int j = 0; j < T; ++j) io.st_out(0u, j, acc[j]);
+}
+
+// NBUF = 2: ping-pong LDS buffers (one barrier per exchange); NBUF = 1: one
+// buffer and an extra barrier before each reuse.
+template <int TL, int PQ, bool ACC, int NBUF, class IO>
+__device__ __forceinline__ void body_encode_tile(IO &io, int q, int p, uint32_t *tile, uint32_t chunks,
+                                                 const MulTab *tw) {
+    using TT = Tile<TL, PQ>;
+    constexpr int T = TT::N, Q = TT::Q, GB = TT::GB, RB = TT::RB;
+    uint32_t acc[GB][Q], y[GB][Q], x[16];
+#pragma unroll
+    for (int m = 0; m < GB; ++m)
+#pragma unroll
+        for (int h = 0; h < Q; ++h) acc[m][h] = ACC ? io.ld_out((uint32_t)(q * GB + m), h * 16) : 0u;
+    for (uint32_t c = 0; c < chunks; ++c) {
+        const uint32_t B = (c + 1) * T;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) x[j] = io.ld_in(c * T + q * 16, j);
+        run_levels<4, 0, 0, 4, true, false>(x, B + q * 16, tw);
+        uint32_t *tb = tile + (NBUF == 2 ? (c & 1) * TT::WORDS : 0);
+        if (NBUF == 1 && c > 0) __syncthreads();
+        TT::put_a(tb, x, q, p);
+        __syncthreads();
+        TT::get_b(tb, y, q, p);
+#pragma unroll
+        for (int m = 0; m < GB; ++m) {
+            run_levels<RB, 4, 4, TL, true, false>(y[m], B, tw);
+#pragma unroll
+            for (int h = 0; h < Q; ++h) acc[m][h] ^= y[m][h];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < GB; ++m) run_levels<RB, 4, 4, TL, false, true>(acc[m], 0u, tw);
+    uint32_t *tb = tile + (NBUF == 2 ? (chunks & 1) * TT::WORDS : 0);
+    if (NBUF == 1) __syncthreads();
+    TT::put_b(tb, acc, q, p);
+    __syncthreads();
+    TT::get_a(tb, x, q, p);
+    run_levels<4, 0, 0, 4, false, false>(x, q * 16, tw);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) io.st_out((uint32_t)(q * 16), j, x[j]);
+}
+
+// ---------------------------------------------------------------------------
+// Encode extension for k = 2^KL with T not a power of two: coefficients
+// d = IFFT_k(message, omega_{n-k}); parity block b = FFT_k(d, omega_{b*k}).
+template <int KL, class IO>
+__device__ __forceinline__ void body_encode_kpow2_small(IO &io, uint32_t blocks, uint32_t msg_base,
+                                                        const MulTab *tw) {
+    constexpr int K = 1 << KL;
+    uint32_t d[K], x[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) d[j] = io.ld_in(0u, j);
+    run_levels<KL, 0, 0, KL, true, false>(d, msg_base, tw);
+    for (uint32_t b = 0; b < blocks; ++b) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) x[j] = d[j];
+        run_levels<KL, 0, 0, KL, false, false>(x, b * K, tw);
+#pragma unroll
+        for (int j = 0; j < K; ++j) io.st_out(b * K, j, x[j]);
+    }
+}
+
+template <int KL, int PQ, int NBUF, class IO>
+__device__ __forceinline__ void body_encode_kpow2_tile(IO &io, int q, int p, uint32_t *tile,
+                                                       uint32_t blocks, uint32_t msg_base,
+                                                       const MulTab *tw) {
+    using TT = Tile<KL, PQ>;
+    constexpr int K = TT::N, Q = TT::Q, GB = TT::GB, RB = TT::RB;
+    uint32_t x[16], d[GB][Q], y[GB][Q];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) x[j] = io.ld_in((uint32_t)(q * 16), j);
+    run_levels<4, 0, 0, 4, true, false>(x, msg_base + q * 16, tw);
+    TT::put_a(tile, x, q, p);
+    __syncthreads();
+    TT::get_b(tile, d, q, p);
+#pragma unroll
+    for (int m = 0; m < GB; ++m) run_levels<RB, 4, 4, KL, true, false>(d[m], msg_base, tw);
+    for (uint32_t b = 0; b < blocks; ++b) {
+#pragma unroll
+        for (int m = 0; m < GB; ++m) {
+#pragma unroll
+            for (int h = 0; h < Q; ++h) y[m][h] = d[m][h];
+            run_levels<RB, 4, 4, KL, false, false>(y[m], b * K, tw);
+        }
+        uint32_t *tb = tile + (NBUF == 2 ? ((b + 1) & 1) * TT::WORDS : 0);
+        if (NBUF == 1) __syncthreads();
+        TT::put_b(tb, y, q, p);
+        __syncthreads();
+        TT::get_a(tb, x, q, p);
+        run_levels<4, 0, 0, 4, false, false>(x, b * K + q * 16, tw);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) io.st_out(b * K + q * 16, j, x[j]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Recover by LCH formal-derivative erasure decoding (all n points):
+//   u_i = r_i * Pi(omega_i)            (0 at erased i)
+//   D   = IFFT_n(u, 0)                 (coefficients of f*Pi, deg < n)
+//   D'[i] = xor over b with bit b of i clear of D[i | 2^b]
+//   f(omega_p) = FFT_n(D', 0)[p] / Pi'(omega_p)  for erased p
+// and, as the reference does (it writes the error value, rs/src/codec.rs:560-566),
+// the erased shard's previous content g_p is XORed in: out_p = c_p ^ g_p.
+// Input and output rows are the same codeword.
+template <int NL, class IO, class TABS>
+__device__ __forceinline__ void body_recover_small(IO &io, const TABS &tabs, const MulTab *tw) {
+    constexpr int N = 1 << NL;
+    uint32_t x[N], orig[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        orig[i] = io.ld_in(0u, i);
+        const uint32_t u = gmul4(orig[i], tabs.tab(i));
+        x[i] = tabs.erased(i) ? 0u : u;
+    }
+    run_levels<NL, 0, 0, NL, true, true>(x, 0u, tw);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        uint32_t a = 0;
+#pragma unroll
+        for (int b = 0; b < NL; ++b)
+            if (!((i >> b) & 1)) a ^= x[i | (1 << b)];
+        x[i] = a;
+    }
+    run_levels<NL, 0, 0, NL, false, true>(x, 0u, tw);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+        if (tabs.erased(i)) io.st_out(0u, i, gmul4(x[i], tabs.tab(i)) ^ orig[i]);
+}
+
+template <int NL, int PQ, int NBUF, class IO, class TABS>
+__device__ __forceinline__ void body_recover_tile(IO &io, const TABS &tabs, int q, int p, uint32_t *tile,
+                                                  const MulTab *tw) {
+    using TT = Tile<NL, PQ>;
+    constexpr int Q = TT::Q, GB = TT::GB, RB = TT::RB;
+    uint32_t x[16], orig[16], y[GB][Q];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        orig[j] = io.ld_in((uint32_t)(q * 16), j);
+        const int s = q * 16 + j;
+        const uint32_t u = gmul4(orig[j], tabs.tab(s));
+        x[j] = tabs.erased(s) ? 0u : u;
+    }
+    run_levels<4, 0, 0, 4, true, false>(x, q * 16, tw);
+    uint32_t *t0 = tile;
+    uint32_t *t1 = tile + (NBUF - 1) * TT::WORDS;
+    TT::put_a(t0, x, q, p);
+    __syncthreads();
+    TT::get_b(t0, y, q, p);
+#pragma unroll
+    for (int m = 0; m < GB; ++m) run_levels<RB, 4, 4, NL, true, true>(y[m], 0u, tw);
+    if constexpr (NBUF == 1) __syncthreads();
+    TT::put_b(t1, y, q, p);
+    __syncthreads();
+    // formal derivative: bits 4.. from registers, bits 0..3 from LDS
+#pragma unroll
+    for (int m = 0; m < GB; ++m) {
+        const int c = q * GB + m;
+#pragma unroll
+        for (int h = 0; h < Q; ++h) {
+            uint32_t a = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t v = t1[(h * 16 + (c | (1 << b))) * PQ + p];
+                a ^= ((c >> b) & 1) ? 0u : v;
+            }
+#pragma unroll
+            for (int b = 0; b < RB; ++b)
+                if (!((h >> b) & 1)) a ^= y[m][h | (1 << b)];
+            y[m][h] = a;
+        }
+        run_levels<RB, 4, 4, NL, false, true>(y[m], 0u, tw);
+    }
+    if constexpr (NBUF == 1) __syncthreads();
+    TT::put_b(t0, y, q, p);
+    __syncthreads();
+    TT::get_a(t0, x, q, p);
+    run_levels<4, 0, 0, 4, false, false>(x, q * 16, tw);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int s = q * 16 + j;
+        if (tabs.erased(s)) io.st_out((uint32_t)(q * 16), j, gmul4(x[j], tabs.tab(s)) ^ orig[j]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Codec::recompute_data_from_parity_sharded (rs/src/codec.rs:321-340):
+// message = FFT_T(IFFT_T(parity, 0), omega_T), n = 2T.  Input rows: parity
+// shards 0..T-1; output rows: the same buffer, shards T..2T-1.
+template <int TL, class IO>
+__device__ __forceinline__ void body_recompute_small(IO &io, const MulTab *tw) {
+    constexpr int T = 1 << TL;
+    uint32_t x[T];
+#pragma unroll
+    for (int j = 0; j < T; ++j) x[j] = io.ld_in(0u, j);
+    run_levels<TL, 0, 0, TL, true, true>(x, 0u, tw);
+    run_levels<TL, 0, 0, TL, false, false>(x, (uint32_t)T, tw);
+#pragma unroll
+    for (int j = 0; j < T; ++j) io.st_out((uint32_t)T, j, x[j]);
+}
+
+template <int TL, int PQ, class IO>
+__device__ __forceinline__ void body_recompute_tile(IO &io, int q, int p, uint32_t *tile, const MulTab *tw) {
+    using TT = Tile<TL, PQ>;
+    constexpr int T = TT::N, Q = TT::Q, GB = TT::GB, RB = TT::RB;
+    uint32_t x[16], y[GB][Q];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) x[j] = io.ld_in((uint32_t)(q * 16), j);
+    run_levels<4, 0, 0, 4, true, false>(x, q * 16, tw);
+    TT::put_a(tile, x, q, p);
+    __syncthreads();
+    TT::get_b(tile, y, q, p);
+#pragma unroll
+    for (int m = 0; m < GB; ++m) {
+        run_levels<RB, 4, 4, TL, true, true>(y[m], 0u, tw);
+        run_levels<RB, 4, 4, TL, false, false>(y[m], (uint32_t)T, tw);
+    }
+    uint32_t *t1 = tile + TT::WORDS;
+    TT::put_b(t1, y, q, p);
+    __syncthreads();
+    TT::get_a(t1, x, q, p);
+    run_levels<4, 0, 0, 4, false, false>(x, T + q * 16, tw);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) io.st_out((uint32_t)(T + q * 16), j, x[j]);
+}
+
+// ---------------------------------------------------------------------------
+// Recover: per-job multiplier tables.  For shard i, Pi(omega_i) =
+// prod_{p in E} (omega_i + omega_p); erased shards get 1 / Pi'(omega_i) with
+// Pi'(omega_i) = prod_{p in E, p != i} (omega_i + omega_p) -- the
+// reference's `denoms` (rs/src/codec.rs:457-462).  One workgroup per job.
+__device__ __forceinline__ void recover_prep_body(RecPrep *__restrict__ out, uint32_t n, uint32_t e,
+                                                  const uint8_t *pos, const uint8_t *__restrict__ gtab) {
+    __shared__ uint8_t s_exp[512], s_log[256], s_sub[256], s_er[256], s_pts[256];
+    const uint32_t t = threadIdx.x;
+    s_exp[t] = gtab[t];
+    s_exp[t + 256] = gtab[t + 256];
+    s_log[t] = gtab[512 + t];
+    s_sub[t] = gtab[768 + t];
+    s_er[t] = 0;
+    __syncthreads();
+    if (t < e) {
+        s_er[pos[t]] = 1;
+        s_pts[t] = s_sub[pos[t]];
+    }
+    __syncthreads();
+    if (t < n) {
+        const uint8_t x = s_sub[t];
+        uint32_t lsum = 0;
+        for (uint32_t i = 0; i < e; ++i) {
+            const uint8_t dlt = x ^ s_pts[i];
+            if (dlt) lsum += s_log[dlt];
+        }
+        lsum %= 255u;
+        const uint32_t er = s_er[t];
+        const uint32_t lv = er ? (255u - lsum) % 255u : lsum;
+        uint8_t tb[3][8];
+        for (int v = 0; v < 8; ++v) {
+            const uint32_t m0 = (uint32_t)v, m1 = (uint32_t)v << 3, m2 = (uint32_t)v << 6;
+            tb[0][v] = m0 ? s_exp[lv + s_log[m0]] : 0;
+            tb[1][v] = m1 ? s_exp[lv + s_log[m1]] : 0;
+            tb[2][v] = (v < 4 && m2) ? s_exp[lv + s_log[m2 & 0xff]] : 0;
+        }
+        MulTab m;
+        for (int k = 0; k < 3; ++k) {
+            m.w[2 * k] = tb[k][0] | (tb[k][1] << 8) | (tb[k][2] << 16) | ((uint32_t)tb[k][3] << 24);
+            m.w[2 * k + 1] = tb[k][4] | (tb[k][5] << 8) | (tb[k][6] << 16) | ((uint32_t)tb[k][7] << 24);
+        }
+        out->tab[t] = m;
+        out->erased[t] = (uint8_t)er;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// I/O policies and multiplier sources.
+
+// One codeword per launch: buffer resources, scalar per-shard offsets.
+template <bool AL>
+struct BufIO {
+    Rows in, out;
+    uint32_t L, off, nb;
+    __device__ __forceinline__ uint32_t ld_in(uint32_t g, int j) const {
+        return ldq<AL>(in, off + g * L, (uint32_t)j * L, nb);
+    }
+    __device__ __forceinline__ uint32_t ld_out(uint32_t g, int j) const {
+        return ldq<AL>(out, off + g * L, (uint32_t)j * L, nb);
+    }
+    __device__ __forceinline__ void st_out(uint32_t g, int j, uint32_t v) const {
+        stq<AL>(out, off + g * L, (uint32_t)j * L, v, nb);
+    }
+};
+
+template <bool AL>
+__device__ __forceinline__ BufIO<AL> make_bufio(const void *in, uint32_t in_bytes, void *out,
+                                                uint32_t out_bytes, uint32_t quad, uint32_t L) {
+    BufIO<AL> io;
+    io.in = make_rows(in, in_bytes);
+    io.out = make_rows(out, out_bytes);
+    io.L = L;
+    const uint32_t col = quad * 4u;
+    io.nb = col < L ? ((L - col) < 4u ? (L - col) : 4u) : 0u;
+    io.off = (AL && io.nb == 0u) ? 0u : col;
+    return io;
+}
+
+// Per-shard multipliers staged in LDS (one codeword per workgroup).
+struct LdsTabs {
+    const MulTab *t;
+    const uint8_t *e;
+    __device__ __forceinline__ MulTab tab(int s) const { return lds_tab(t, (uint32_t)s); }
+    __device__ __forceinline__ bool erased(int s) const { return e[s] != 0; }
+};
+
+}  // namespace ars

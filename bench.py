@@ -89,6 +89,79 @@ def measured_traffic(kernel):
         return None
 
 
+def bench_batched(ars, torch, np, stream, count=4096, seed=44, reps=3):
+    """Mixed sizes, one persistent launch vs one launch per codeword: `count`
+    codewords with n ~ U{32,64,128,256}, k = n/2, shard_len log-uniform in
+    [64 B, 64 KiB] (multiple of 64); each is encoded, then recovers
+    e ~ U{0..T} random erasures."""
+    rng = np.random.default_rng(seed)
+    ns = rng.choice([32, 64, 128, 256], count)
+    Ls = (64 * np.maximum(1, np.round(2.0 ** rng.uniform(6, 16, count) / 64))).astype(np.int64)
+    sizes = ns * Ls
+    offs = np.zeros(count, np.int64)
+    offs[1:] = np.cumsum((sizes + 255) // 256 * 256)[:-1]
+    total = int(offs[-1] + sizes[-1])
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    pool = torch.randint(0, 256, (total,), dtype=torch.uint8, device="cuda", generator=gen)
+    enc, rec = [], []
+    enc_bytes = rec_bytes = 0
+    for i in range(count):
+        n, L, o = int(ns[i]), int(Ls[i]), int(offs[i])
+        k = n // 2
+        T = n - k
+        enc.append({"op": "encode", "n": n, "k": k, "shard_len": L,
+                    "message": pool[o + T * L:o + n * L], "parity": pool[o:o + T * L]})
+        e = int(rng.integers(0, T + 1))
+        rec.append({"op": "recover", "n": n, "k": k, "shard_len": L, "codeword": pool[o:o + n * L],
+                    "positions": rng.permutation(n)[:e]})
+        enc_bytes += n * L
+        rec_bytes += (n + e) * L if e else 0
+    earr, ekeep = ars.Batch.make_jobs(enc)
+    rarr, rkeep = ars.Batch.make_jobs(rec)
+    b = ars.Batch(count)
+    ctx = b.ctx
+
+    def batched():
+        b.run_jobs(earr, count, stream)
+        b.run_jobs(rarr, count, stream)
+
+    def plain():
+        ars.lib().ars_run_jobs_plain(ctx.handle, earr, count, stream.cuda_stream)
+        ars.lib().ars_run_jobs_plain(ctx.handle, rarr, count, stream.cuda_stream)
+
+    def timed(fn, r):
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        z = torch.cuda.Event(enable_timing=True)
+        w0 = time.perf_counter()
+        a.record(stream)
+        for _ in range(r):
+            fn()
+        z.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(z) / r * 1e-3, (time.perf_counter() - w0) / r
+
+    s_b, w_b = timed(batched, reps)
+    s_p, w_p = timed(plain, reps)
+    byts = enc_bytes + rec_bytes
+    info = b.info()
+    b.close()
+    del pool
+    return {
+        "workload": f"{count} codewords, n~U{{32,64,128,256}}, k=n/2, L log-uniform 64 B-64 KiB, "
+                    "encode then recover e~U{0..T}",
+        "bytes": byts, "data_bytes": int(sizes.sum()),
+        "batched_GBps": round(byts / max(s_b, w_b) / 1e9, 1),
+        "batched_ms": round(max(s_b, w_b) * 1e3, 3),
+        "plain_GBps": round(byts / max(s_p, w_p) / 1e9, 1),
+        "plain_ms": round(max(s_p, w_p) * 1e3, 3),
+        "speedup": round(max(s_p, w_p) / max(s_b, w_b), 2),
+        "items_last_run": info["items"], "grid": info["grid"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +169,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every figure is "
@@ -221,6 +295,9 @@ def main():
                            "frac_hbm": round(copy_gbps / HBM_PEAK_GBPS, 4)},
         "wall_s_timed_region": round(wall, 4),
     }
+    if not args.no_batched:
+        del plan
+        out["batched"] = bench_batched(ars, torch, np, stream)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, k, L, e, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -106,6 +106,56 @@ int ars_recover_with_plan(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_recei
 int ars_recompute_data(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received,
                        size_t shard_len, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Batches: many codewords of uneven (n, k), shard lengths and erasure
+ * patterns in one persistent kernel launch, replayed from a hipGraph.
+ * No counterpart in the reference (it processes one codeword per call); each
+ * job has exactly the semantics of ars_encode / ars_recover on its own.
+ * ------------------------------------------------------------------------ */
+#define ARS_OP_ENCODE 1u
+#define ARS_OP_RECOVER 2u
+
+typedef struct ars_job {
+    uint32_t op;               /* ARS_OP_ENCODE or ARS_OP_RECOVER */
+    uint32_t n, k;
+    uint32_t flags;            /* encode: ARS_ENC_ACCUMULATE */
+    uint64_t shard_len;
+    const uint8_t *d_message;  /* encode: k * shard_len bytes (device) */
+    uint8_t *d_parity;         /* encode: (n - k) * shard_len bytes (device) */
+    uint8_t *d_codeword;       /* recover: n * shard_len bytes (device), in place */
+    const uint8_t *positions;  /* recover: e erased shard indices (host) */
+    uint32_t e;
+    int32_t status;            /* out: what ars_encode / ars_recover would return */
+} ars_job;
+
+typedef struct ars_batch ars_batch;
+
+typedef struct ars_batch_info {
+    uint32_t jobs, items, segments, plans, fallback_jobs, grid;
+    uint64_t algorithmic_bytes;  /* sum over jobs of read + written shard bytes */
+} ars_batch_info;
+
+#define ARS_BATCH_NO_GRAPH 1u  /* launch the kernels directly instead of replaying the graph */
+
+/* A batch object owns a device arena for descriptors and erasure plans and a
+ * captured graph; it serves one run at a time (runs on one object are
+ * ordered on their streams; use several objects for overlap). */
+int ars_batch_create(ars_ctx *ctx, uint32_t max_jobs, ars_batch **out);
+int ars_batch_destroy(ars_batch *b);
+/* Validates every job (setting its status), then launches all valid jobs as
+ * one batch on `stream`, asynchronously.  Jobs the batched kernel cannot take
+ * (shard_len % 4 != 0 or unaligned pointers) run as plain calls on the same
+ * stream.  Returns ARS_OK or a negative code for a failure of the batch
+ * itself (the per-job failures are in the statuses). */
+int ars_batch_run(ars_batch *b, ars_job *jobs, uint32_t count, uint32_t flags, void *stream);
+int ars_batch_info_get(const ars_batch *b, ars_batch_info *info);
+/* Persistent workgroups per compute unit (0 = occupancy maximum); the
+ * autotuner sets this. */
+int ars_batch_set_blocks_per_cu(ars_batch *b, uint32_t blocks_per_cu);
+/* The same jobs as one ars_encode / ars_recover call each, in order, on
+ * `stream` (the one-codeword-per-launch form the batch is measured against). */
+int ars_run_jobs_plain(ars_ctx *ctx, ars_job *jobs, uint32_t count, void *stream);
+
 /* Host-slice variants: same semantics on host buffers, synchronous; they
  * stage through device memory (PCIe-bound, not the HBM roofline path). */
 int ars_encode_host(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *message,
