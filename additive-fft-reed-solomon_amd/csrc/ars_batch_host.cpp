@@ -296,28 +296,34 @@ int ars_batch_run(ars_batch *b, ars_job *jobs, uint32_t count, uint32_t flags, v
     //    whole tiles, small ones packed side by side
     std::stable_sort(planned.begin(), planned.end(),
                      [](const Planned &x, const Planned &y) { return x.key < y.key; });
+    // (a packed item's segments must be contiguous, so the large codewords
+    //  of a class are emitted first, then its small ones are packed)
     size_t pi = 0;
     while (pi < planned.size()) {
         const ClassKey key = planned[pi].key;
+        size_t pe = pi;
+        while (pe < planned.size() && planned[pe].key == key) ++pe;
         const uint32_t lanes = ars::batch_lanes((int)key.tl);
         const uint32_t maxseg = ars::batch_max_segs(key.kind, (int)key.tl);
         const uint32_t meta = key.kind | (key.tl << 8) | (key.acc << 16);
+        for (size_t t = pi; t < pe; ++t) {
+            const uint32_t jid = planned[t].job;
+            const ars::BJob &bj = b->jobs[jid];
+            if (!(maxseg == 1 || bj.quads >= lanes)) continue;
+            const uint32_t tiles = (bj.quads + lanes - 1) / lanes;
+            for (uint32_t u = 0; u < tiles; u += kRepMax) {
+                const uint32_t rep = std::min(kRepMax, tiles - u);
+                b->items.push_back({meta, key.arg, (uint32_t)b->segs.size(), 1u | (rep << 16), bj.in, bj.out});
+                b->segs.push_back({jid, u * lanes, 0u, lanes});
+            }
+        }
         int32_t open = -1;  // packed item being filled
         uint32_t used = 0;
-        for (; pi < planned.size() && planned[pi].key == key; ++pi) {
-            const uint32_t jid = planned[pi].job;
+        for (size_t t = pi; t < pe; ++t) {
+            const uint32_t jid = planned[t].job;
             const ars::BJob &bj = b->jobs[jid];
             const uint32_t quads = bj.quads;
-            if (maxseg == 1 || quads >= lanes) {
-                const uint32_t tiles = (quads + lanes - 1) / lanes;
-                for (uint32_t t = 0; t < tiles; t += kRepMax) {
-                    const uint32_t rep = std::min(kRepMax, tiles - t);
-                    b->items.push_back({meta, key.arg, (uint32_t)b->segs.size(), 1u | (rep << 16),
-                                        bj.in, bj.out});
-                    b->segs.push_back({jid, t * lanes, 0u, lanes});
-                }
-                continue;
-            }
+            if (maxseg == 1 || quads >= lanes) continue;
             bool fits = open >= 0 && used + quads <= lanes &&
                         (b->items[open].nseg_rep & 0xffffu) < maxseg;
             if (fits) {
@@ -335,6 +341,7 @@ int ars_batch_run(ars_batch *b, ars_job *jobs, uint32_t count, uint32_t flags, v
             b->items[open].nseg_rep += 1u;
             used += quads;
         }
+        pi = pe;
     }
 
     // 3. arena layout: header | items | segs | jobs | preps | positions | plans

@@ -122,7 +122,7 @@ def test_batch_tiny_codewords_packed(ars):
 
 def test_batch_statuses_and_fallback(ars):
     rng = np.random.default_rng(47)
-    n, k, L = 64, 32, 100  # L % 4 != 0 -> plain-call fallback
+    n, k, L = 64, 32, 102  # L % 4 != 0 -> plain-call fallback
     msg = rng.integers(0, 256, k * L, dtype=np.uint8)
     cw = np.concatenate([O.encode(n, n - k, msg, L), msg])
     rx = cw.copy().reshape(n, L)
