@@ -42,3 +42,45 @@ def test_validation_errors_without_gpu():
     # null context -> ARS_E_NULL
     assert lib.ars_encode(None, 64, 32, 1, 1, 64, 0, None) == ars.ARS_E_NULL
     assert lib.ars_recover(None, 64, 32, 1, 64, None, 0, None) == ars.ARS_E_NULL
+
+
+def test_cpp_mirror_compiles_and_reports_errors(tmp_path):
+    """include/ars_codec.hpp (the C++ Rs<N, T> mirror) compiles against
+    include/ars.h, links libars.so, and turns ABI errors into ars::Error.
+    On a host without a GPU, context creation fails with ARS_E_HIP; with a
+    GPU it succeeds.  Static checks reject N, T outside the reference's
+    contract (rs/src/lib.rs:16-19)."""
+    import shutil
+    import subprocess
+    ars = _pkg.load()
+    if not os.path.exists(ars.LIB_PATH) or not shutil.which("g++"):
+        pytest.skip("libars.so or g++ missing")
+    src = tmp_path / "m.cpp"
+    src.write_text(r'''
+#include "ars_codec.hpp"
+#include <cstdio>
+int main() {
+    if (ars_encode(nullptr, 64, 32, nullptr, nullptr, 64, 0, nullptr) != ARS_E_NULL) return 2;
+    try {
+        ars::Rs<64, 32> rs;
+        std::puts("context ok");
+    } catch (const ars::Error &e) {
+        std::printf("error %d\n", e.code);
+        if (e.code != ARS_E_HIP) return 3;
+    }
+    return 0;
+}
+''')
+    libdir = os.path.dirname(ars.LIB_PATH)
+    exe = tmp_path / "m"
+    r = subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                        "-L", libdir, "-lars", f"-Wl,-rpath,{libdir}",
+                        "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.stdout, r.stderr)
+    bad = tmp_path / "bad.cpp"
+    bad.write_text('#include "ars_codec.hpp"\nint main(){ ars::Rs<64, 48> rs; (void)rs; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(bad)],
+                       capture_output=True, text=True)
+    assert r.returncode != 0 and "power of two" in r.stderr
