@@ -63,6 +63,15 @@ SIGNATURES = {
     "ars_batch_info_get": (_i, [_vp, _vp]),
     "ars_batch_set_blocks_per_cu": (_i, [_vp, _u32]),
     "ars_run_jobs_plain": (_i, [_vp, _vp, _u32, _vp]),
+    "ars_server_create": (_i, [_vp, _vp, ctypes.POINTER(_vp)]),
+    "ars_server_destroy": (_i, [_vp]),
+    "ars_server_submit": (_i, [_vp, _vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "ars_server_wait": (_i, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32),
+                             ctypes.POINTER(ctypes.c_double)]),
+    "ars_server_drain": (_i, [_vp]),
+    "ars_server_stats_get": (_i, [_vp, _vp]),
+    "ars_server_stats_reset": (_i, [_vp]),
+    "ars_server_replay": (_i, [_vp, _vp, _vp, _u32, _vp, _vp]),
 }
 
 ARS_OP_ENCODE = 1
@@ -250,6 +259,103 @@ class Batch:
         """Run a prepared ctypes Job array (no per-call Python conversion)."""
         return _check(lib().ars_batch_run(self.handle, arr, count, 0 if graph else ARS_BATCH_NO_GRAPH,
                                           _stream_ptr(stream)))
+
+
+class ServerConfig(ctypes.Structure):
+    """struct ars_server_config (include/ars.h)."""
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("max_batch_jobs", ctypes.c_uint32),
+                ("pipeline_depth", ctypes.c_uint32)]
+
+
+class Request(ctypes.Structure):
+    """struct ars_request (include/ars.h)."""
+    _fields_ = [("op", ctypes.c_uint32), ("n", ctypes.c_uint32), ("k", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32), ("shard_len", ctypes.c_uint64),
+                ("positions", ctypes.c_void_p), ("e", ctypes.c_uint32), ("pad", ctypes.c_uint32),
+                ("host_in", ctypes.c_void_p), ("host_out", ctypes.c_void_p)]
+
+
+class ServerStats(ctypes.Structure):
+    """struct ars_server_stats (include/ars.h)."""
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("rejected", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("algorithmic_bytes", ctypes.c_uint64), ("p50_latency_s", ctypes.c_double),
+                ("p99_latency_s", ctypes.c_double), ("max_latency_s", ctypes.c_double),
+                ("mean_batch_jobs", ctypes.c_double), ("first_arrival_s", ctypes.c_double),
+                ("last_completion_s", ctypes.c_double)]
+
+
+class Server:
+    """Continuous-batching request loop (ars_server_*)."""
+
+    def __init__(self, hbm_budget=64 << 30, max_batch_jobs=1024, pipeline_depth=2, ctx=None):
+        self.ctx = ctx or default_context(0)
+        cfg = ServerConfig(hbm_budget, max_batch_jobs, pipeline_depth)
+        h = ctypes.c_void_p()
+        _check(lib().ars_server_create(self.ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().ars_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def request(op, n, k, shard_len, positions=(), host_in=None, host_out=None, keep=None):
+        r = Request()
+        r.op = ARS_OP_ENCODE if op == "encode" else ARS_OP_RECOVER
+        r.n, r.k, r.shard_len = n, k, shard_len
+        if r.op == ARS_OP_RECOVER:
+            pos = np.ascontiguousarray(np.asarray(positions, dtype=np.uint8))
+            if keep is not None:
+                keep.append(pos)
+            r.positions = pos.ctypes.data if len(pos) else None
+            r.e = len(pos)
+        if host_in is not None:
+            r.host_in = _ptr(host_in)[0]
+        if host_out is not None:
+            r.host_out = _ptr(host_out)[0]
+        return r
+
+    def submit(self, req):
+        t = ctypes.c_uint64()
+        _check(lib().ars_server_submit(self.handle, ctypes.byref(req), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        st = ctypes.c_int32()
+        lat = ctypes.c_double()
+        _check(lib().ars_server_wait(self.handle, ticket, ctypes.byref(st), ctypes.byref(lat)))
+        return st.value, lat.value
+
+    def drain(self):
+        _check(lib().ars_server_drain(self.handle))
+
+    def stats(self, reset=False):
+        o = ServerStats()
+        _check(lib().ars_server_stats_get(self.handle, ctypes.byref(o)))
+        if reset:
+            _check(lib().ars_server_stats_reset(self.handle))
+        return {f: getattr(o, f) for f, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        _check(lib().ars_server_stats_reset(self.handle))
+
+    def replay(self, reqs, arrivals):
+        """reqs: ctypes array of Request; arrivals: seconds after the call."""
+        count = len(arrivals)
+        arr = np.ascontiguousarray(np.asarray(arrivals, dtype=np.float64))
+        st = np.zeros(count, np.int32)
+        lat = np.zeros(count, np.float64)
+        _check(lib().ars_server_replay(self.handle, reqs, arr.ctypes.data, count, st.ctypes.data,
+                                       lat.ctypes.data))
+        return st, lat
 
 
 _default_ctx = {}

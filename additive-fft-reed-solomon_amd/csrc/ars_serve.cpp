@@ -1,0 +1,425 @@
+// ars_serve.cpp -- the serving half of the C ABI: a request loop with
+// continuous batching over the batched persistent kernel.
+//
+// Requests are submitted from any thread.  One dispatcher thread:
+//   1. retires finished batches (event query): statuses, latencies, pool space;
+//   2. admits waiting requests in arrival order while the HBM budget (a
+//      device pool holding each admitted request's codeword) has room --
+//      a request that does not fit waits, later smaller ones may pass it;
+//   3. as soon as one of `pipeline_depth` batch slots (batch object + stream)
+//      is free, launches everything admitted (up to max_batch_jobs) as one
+//      batch, so requests arriving while others run join the next batch and
+//      overlap with it on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/ars.h"
+#include "ars_ctx.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr size_t kPoolAlign = 256;
+
+// First-fit allocator over [0, cap) with coalescing frees.
+class Pool {
+  public:
+    explicit Pool(size_t cap) { free_[0] = cap; }
+    bool alloc(size_t bytes, size_t &off) {
+        bytes = (bytes + kPoolAlign - 1) & ~(kPoolAlign - 1);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                off = it->first;
+                const size_t rest = it->second - bytes;
+                free_.erase(it);
+                if (rest) free_[off + bytes] = rest;
+                used_ += bytes;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = (bytes + kPoolAlign - 1) & ~(kPoolAlign - 1);
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t used() const { return used_; }
+
+  private:
+    std::map<size_t, size_t> free_;
+    size_t used_ = 0;
+};
+
+struct Req {
+    uint64_t ticket = 0;
+    ars_request r{};
+    std::vector<uint8_t> pos;
+    size_t bytes = 0, off = 0;
+    uint64_t alg_bytes = 0;
+    Clock::time_point t_submit, t_done;
+    int32_t status = 0;
+    bool done = false;
+};
+
+struct Slot {
+    ars_batch *batch = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    std::vector<Req *> reqs;
+    std::vector<ars_job> jobs;
+    bool busy = false;
+};
+
+}  // namespace
+
+struct ars_server {
+    ars_ctx *ctx = nullptr;
+    ars_server_config cfg{};
+    uint8_t *d_pool = nullptr;
+    std::unique_ptr<Pool> pool;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Req *> waiting;
+    std::vector<Req *> admitted;
+    std::unordered_map<uint64_t, Req *> tickets;
+    std::vector<Slot> slots;
+    std::thread disp;
+    bool stop = false;
+    uint64_t next_ticket = 1, outstanding = 0;
+    Clock::time_point t0;
+    // statistics since the last reset
+    std::vector<double> lat;
+    uint64_t submitted = 0, completed = 0, rejected = 0, batches = 0, batch_jobs = 0, alg = 0;
+    double first_arrival = -1, last_completion = 0;
+};
+
+namespace {
+
+double secs(const ars_server *s, Clock::time_point t) {
+    return std::chrono::duration<double>(t - s->t0).count();
+}
+
+void launch_slot(ars_server *s, Slot &sl) {
+    sl.jobs.assign(sl.reqs.size(), ars_job{});
+    for (size_t i = 0; i < sl.reqs.size(); ++i) {
+        Req *q = sl.reqs[i];
+        const ars_request &r = q->r;
+        const size_t L = (size_t)r.shard_len, T = r.n - r.k;
+        uint8_t *d = s->d_pool + q->off;
+        ars_job &j = sl.jobs[i];
+        j.op = r.op;
+        j.n = r.n;
+        j.k = r.k;
+        j.flags = r.flags;
+        j.shard_len = r.shard_len;
+        if (r.op == ARS_OP_ENCODE) {
+            j.d_parity = d;
+            j.d_message = d + T * L;
+            if (r.host_in) hipMemcpyAsync(d + T * L, r.host_in, r.k * L, hipMemcpyHostToDevice, sl.stream);
+        } else {
+            j.d_codeword = d;
+            j.positions = q->pos.data();
+            j.e = r.e;
+            if (r.host_in) hipMemcpyAsync(d, r.host_in, r.n * L, hipMemcpyHostToDevice, sl.stream);
+        }
+    }
+    int rc = ars_batch_run(sl.batch, sl.jobs.data(), (uint32_t)sl.jobs.size(), 0, sl.stream);
+    for (size_t i = 0; i < sl.reqs.size(); ++i) {
+        Req *q = sl.reqs[i];
+        const ars_request &r = q->r;
+        q->status = rc < 0 ? rc : sl.jobs[i].status;
+        if (!r.host_out || q->status < 0) continue;
+        const size_t L = (size_t)r.shard_len;
+        uint8_t *d = s->d_pool + q->off;
+        if (r.op == ARS_OP_ENCODE) hipMemcpyAsync(r.host_out, d, (r.n - r.k) * L, hipMemcpyDeviceToHost, sl.stream);
+        else if (q->status == ARS_TRUE) hipMemcpyAsync(r.host_out, d, r.n * L, hipMemcpyDeviceToHost, sl.stream);
+    }
+    hipEventRecord(sl.ev, sl.stream);
+    sl.busy = true;
+}
+
+// Caller holds s->mu.
+void finish(ars_server *s, Req *q, Clock::time_point now) {
+    q->t_done = now;
+    q->done = true;
+    s->lat.push_back(std::chrono::duration<double>(now - q->t_submit).count());
+    s->completed++;
+    if (q->status >= 0) s->alg += q->alg_bytes;
+    s->last_completion = secs(s, now);
+    s->outstanding--;
+}
+
+void dispatcher(ars_server *s) {
+    hipSetDevice(s->ctx->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        bool progress = false;
+        // 1. retire finished batches
+        for (Slot &sl : s->slots) {
+            if (!sl.busy) continue;
+            if (hipEventQuery(sl.ev) != hipSuccess) continue;
+            const Clock::time_point now = Clock::now();
+            for (Req *q : sl.reqs) {
+                s->pool->release(q->off, q->bytes);
+                finish(s, q, now);
+            }
+            sl.reqs.clear();
+            sl.busy = false;
+            progress = true;
+            s->cv_done.notify_all();
+        }
+        // 2. admission against the budget
+        for (auto it = s->waiting.begin(); it != s->waiting.end();) {
+            Req *q = *it;
+            size_t off;
+            if (s->pool->alloc(q->bytes, off)) {
+                q->off = off;
+                s->admitted.push_back(q);
+                it = s->waiting.erase(it);
+                progress = true;
+            } else {
+                ++it;
+            }
+        }
+        // 3. launch on a free slot
+        if (!s->admitted.empty()) {
+            for (Slot &sl : s->slots) {
+                if (sl.busy) continue;
+                const size_t take = std::min<size_t>(s->admitted.size(), s->cfg.max_batch_jobs);
+                sl.reqs.assign(s->admitted.begin(), s->admitted.begin() + take);
+                s->admitted.erase(s->admitted.begin(), s->admitted.begin() + take);
+                s->batches++;
+                s->batch_jobs += take;
+                lk.unlock();
+                launch_slot(s, sl);
+                lk.lock();
+                progress = true;
+                break;
+            }
+        }
+        bool busy = false;
+        for (const Slot &sl : s->slots) busy |= sl.busy;
+        if (s->stop && s->waiting.empty() && s->admitted.empty() && !busy) break;
+        if (!progress) {
+            if (busy) s->cv_work.wait_for(lk, std::chrono::microseconds(20));
+            else s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ars_server_create(ars_ctx *ctx, const ars_server_config *cfg, ars_server **out) {
+    if (!ctx || !out) return ars::set_error(ARS_E_NULL, "null context or out");
+    *out = nullptr;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return ars::hip_error(e, "hipSetDevice");
+    ars_server *s = new (std::nothrow) ars_server;
+    if (!s) return ars::set_error(ARS_E_NOMEM, "server allocation failed");
+    s->ctx = ctx;
+    if (cfg) s->cfg = *cfg;
+    if (!s->cfg.hbm_budget) s->cfg.hbm_budget = 64ull << 30;
+    if (!s->cfg.max_batch_jobs) s->cfg.max_batch_jobs = 1024;
+    if (!s->cfg.pipeline_depth) s->cfg.pipeline_depth = 2;
+    if ((e = hipMalloc(&s->d_pool, s->cfg.hbm_budget)) != hipSuccess) {
+        delete s;
+        return ars::hip_error(e, "hipMalloc(server pool)");
+    }
+    s->pool.reset(new Pool(s->cfg.hbm_budget));
+    s->slots.resize(s->cfg.pipeline_depth);
+    for (Slot &sl : s->slots) {
+        int rc = ars_batch_create(ctx, s->cfg.max_batch_jobs, &sl.batch);
+        if (rc == ARS_OK && (e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess)
+            rc = ars::hip_error(e, "hipStreamCreate");
+        if (rc == ARS_OK && (e = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming)) != hipSuccess)
+            rc = ars::hip_error(e, "hipEventCreate");
+        if (rc != ARS_OK) {
+            ars_server_destroy(s);
+            return rc;
+        }
+    }
+    s->t0 = Clock::now();
+    s->disp = std::thread(dispatcher, s);
+    *out = s;
+    return ARS_OK;
+}
+
+int ars_server_destroy(ars_server *s) {
+    if (!s) return ARS_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->disp.joinable()) s->disp.join();
+    hipSetDevice(s->ctx->device);
+    for (Slot &sl : s->slots) {
+        if (sl.stream) hipStreamSynchronize(sl.stream);
+        if (sl.batch) ars_batch_destroy(sl.batch);
+        if (sl.ev) hipEventDestroy(sl.ev);
+        if (sl.stream) hipStreamDestroy(sl.stream);
+    }
+    for (auto &kv : s->tickets) delete kv.second;
+    hipFree(s->d_pool);
+    delete s;
+    return ARS_OK;
+}
+
+int ars_server_submit(ars_server *s, const ars_request *req, uint64_t *ticket) {
+    if (!s || !req || !ticket) return ars::set_error(ARS_E_NULL, "null server, request or ticket");
+    int rc = ars::check_code(req->n, req->k, (size_t)req->shard_len);
+    if (rc) return rc;
+    if (req->op != ARS_OP_ENCODE && req->op != ARS_OP_RECOVER) return ars::set_error(ARS_E_INVAL, "unknown op");
+    if (req->flags) return ars::set_error(ARS_E_INVAL, "server requests take no flags");
+    if (req->op == ARS_OP_RECOVER && req->e && !req->positions)
+        return ars::set_error(ARS_E_NULL, "positions is null");
+    Req *q = new (std::nothrow) Req;
+    if (!q) return ars::set_error(ARS_E_NOMEM, "request allocation failed");
+    q->r = *req;
+    const size_t L = (size_t)req->shard_len;
+    q->bytes = (size_t)req->n * L;
+    if (q->bytes > s->cfg.hbm_budget) {
+        delete q;
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->rejected++;
+        return ars::set_error(ARS_E_BUDGET, "request larger than the server's HBM budget");
+    }
+    if (req->op == ARS_OP_RECOVER && req->e) {
+        q->pos.assign(req->positions, req->positions + std::min<uint32_t>(req->e, 256));
+        q->r.positions = nullptr;
+        if (req->e > 256) q->r.e = 256;  // e > T anyway: the job reports false
+    }
+    q->alg_bytes = req->op == ARS_OP_ENCODE ? (uint64_t)req->n * L : (uint64_t)(req->n + req->e) * L;
+    q->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        q->ticket = s->next_ticket++;
+        s->tickets[q->ticket] = q;
+        s->waiting.push_back(q);
+        s->submitted++;
+        s->outstanding++;
+        if (s->first_arrival < 0) s->first_arrival = secs(s, q->t_submit);
+        *ticket = q->ticket;
+    }
+    s->cv_work.notify_one();
+    return ARS_OK;
+}
+
+int ars_server_wait(ars_server *s, uint64_t ticket, int32_t *status, double *latency_s) {
+    if (!s) return ars::set_error(ARS_E_NULL, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return ars::set_error(ARS_E_STATE, "unknown or already collected ticket");
+    Req *q = it->second;
+    s->cv_done.wait(lk, [q] { return q->done; });
+    if (status) *status = q->status;
+    if (latency_s) *latency_s = std::chrono::duration<double>(q->t_done - q->t_submit).count();
+    s->tickets.erase(it);
+    delete q;
+    return ARS_OK;
+}
+
+int ars_server_drain(ars_server *s) {
+    if (!s) return ars::set_error(ARS_E_NULL, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [s] { return s->outstanding == 0; });
+    return ARS_OK;
+}
+
+int ars_server_stats_get(ars_server *s, ars_server_stats *o) {
+    if (!s || !o) return ars::set_error(ARS_E_NULL, "null server or stats");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::memset(o, 0, sizeof *o);
+    o->submitted = s->submitted;
+    o->completed = s->completed;
+    o->rejected = s->rejected;
+    o->batches = s->batches;
+    o->algorithmic_bytes = s->alg;
+    o->mean_batch_jobs = s->batches ? (double)s->batch_jobs / (double)s->batches : 0.0;
+    o->first_arrival_s = s->first_arrival;
+    o->last_completion_s = s->last_completion;
+    if (!s->lat.empty()) {
+        std::vector<double> v = s->lat;
+        auto pct = [&v](double p) {
+            const size_t i = std::min(v.size() - 1, (size_t)(p * (double)(v.size() - 1) + 0.5));
+            std::nth_element(v.begin(), v.begin() + i, v.end());
+            return v[i];
+        };
+        o->p50_latency_s = pct(0.50);
+        o->p99_latency_s = pct(0.99);
+        o->max_latency_s = *std::max_element(v.begin(), v.end());
+    }
+    return ARS_OK;
+}
+
+int ars_server_stats_reset(ars_server *s) {
+    if (!s) return ars::set_error(ARS_E_NULL, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->lat.clear();
+    s->submitted = s->completed = s->rejected = s->batches = s->batch_jobs = s->alg = 0;
+    s->first_arrival = -1;
+    s->last_completion = 0;
+    return ARS_OK;
+}
+
+int ars_server_replay(ars_server *s, const ars_request *reqs, const double *arrival_s, uint32_t count,
+                      int32_t *statuses, double *latencies_s) {
+    if (!s || (count && (!reqs || !arrival_s))) return ars::set_error(ARS_E_NULL, "null server or trace");
+    std::vector<uint64_t> tickets(count, 0);
+    std::vector<int> rcs(count, ARS_OK);
+    std::thread gen([&] {
+        const Clock::time_point start = Clock::now();
+        for (uint32_t i = 0; i < count; ++i) {
+            const Clock::time_point at = start + std::chrono::duration_cast<Clock::duration>(
+                                                     std::chrono::duration<double>(arrival_s[i]));
+            while (Clock::now() < at) {
+                const auto left = at - Clock::now();
+                if (left > std::chrono::microseconds(200)) std::this_thread::sleep_for(left - std::chrono::microseconds(100));
+            }
+            rcs[i] = ars_server_submit(s, &reqs[i], &tickets[i]);
+        }
+    });
+    gen.join();
+    int rc = ARS_OK;
+    for (uint32_t i = 0; i < count; ++i) {
+        int32_t st = rcs[i];
+        double lat = 0;
+        if (rcs[i] == ARS_OK) {
+            int w = ars_server_wait(s, tickets[i], &st, &lat);
+            if (w) rc = w;
+        }
+        if (statuses) statuses[i] = st;
+        if (latencies_s) latencies_s[i] = lat;
+    }
+    return rc;
+}
+
+}  // extern "C"

@@ -162,6 +162,45 @@ def bench_batched(ars, torch, np, stream, count=4096, seed=44, reps=3):
     }
 
 
+def bench_served(ars, np, rates=(1000, 2000, 4000, 8000, 12000), per_rate=1500, seed=45):
+    """Poisson arrivals, 70 % encode / 30 % recover, (n,k) ~ U{(64,32),(128,96),
+    (256,128),(256,224)}, shard_len log-uniform 1 KiB-4 MiB (multiple of 64),
+    recover e ~ U{1..T}; 64 GiB HBM admission budget; payloads are
+    device-resident (the pool slot's bytes), so latency is queueing + GPU."""
+    shapes = [(64, 32), (128, 96), (256, 128), (256, 224)]
+    srv = ars.Server(hbm_budget=64 << 30, max_batch_jobs=1024, pipeline_depth=2)
+    out = []
+    for ri, rate in enumerate(rates):
+        rng = np.random.default_rng(seed + ri)
+        reqs = (ars.Request * per_rate)()
+        keep = []
+        for i in range(per_rate):
+            n, k = shapes[rng.integers(4)]
+            L = int(64 * max(1, round(2.0 ** rng.uniform(10, 22) / 64)))
+            if rng.random() < 0.7:
+                reqs[i] = ars.Server.request("encode", n, k, L)
+            else:
+                e = int(rng.integers(1, n - k + 1))
+                reqs[i] = ars.Server.request("recover", n, k, L, positions=rng.permutation(n)[:e], keep=keep)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, per_rate))
+        srv.reset_stats()
+        st, lat = srv.replay(reqs, arrivals)
+        stt = srv.stats()
+        span = stt["last_completion_s"] - stt["first_arrival_s"]
+        out.append({"rate_req_per_s": rate, "requests": per_rate,
+                    "ok": int((st >= 0).sum()),
+                    "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                    "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                    "throughput_GBps": round(stt["algorithmic_bytes"] / span / 1e9, 1),
+                    "achieved_req_per_s": round(per_rate / span, 1),
+                    "mean_batch_jobs": round(stt["mean_batch_jobs"], 2)})
+        print(f"served rate {rate}: {out[-1]}", file=sys.stderr, flush=True)
+    srv.close()
+    return {"workload": "Poisson arrivals, 70% encode / 30% recover, (n,k) in {(64,32),(128,96),"
+                        "(256,128),(256,224)}, L log-uniform 1 KiB-4 MiB, e~U{1..T}, 64 GiB budget",
+            "sweep": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +209,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every figure is "
@@ -298,6 +338,8 @@ def main():
     if not args.no_batched:
         del plan
         out["batched"] = bench_batched(ars, torch, np, stream)
+    if not args.no_served:
+        out["served"] = bench_served(ars, np)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, k, L, e, args.cpu_seconds)
     print(json.dumps(out), flush=True)

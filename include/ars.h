@@ -156,6 +156,66 @@ int ars_batch_set_blocks_per_cu(ars_batch *b, uint32_t blocks_per_cu);
  * `stream` (the one-codeword-per-launch form the batch is measured against). */
 int ars_run_jobs_plain(ars_ctx *ctx, ars_job *jobs, uint32_t count, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Serving: a request loop with continuous batching.  No counterpart in the
+ * reference.  Requests (encode / recover of any supported shape) arrive at
+ * any time from any thread; a dispatcher thread admits them against an HBM
+ * budget (a device pool that holds each admitted request's codeword), groups
+ * whatever is admitted into a batch and launches it on one of
+ * `pipeline_depth` streams as soon as one is free, so requests that arrive
+ * while others run join the next batch and overlap with it on the device.
+ * Each request gets exactly the single-call result.
+ * ------------------------------------------------------------------------ */
+typedef struct ars_server ars_server;
+
+typedef struct ars_server_config {
+    uint64_t hbm_budget;       /* bytes of device pool for request codewords (0 = 64 GiB) */
+    uint32_t max_batch_jobs;   /* jobs per launched batch (0 = 1024) */
+    uint32_t pipeline_depth;   /* batches in flight at once, one stream each (0 = 2) */
+} ars_server_config;
+
+typedef struct ars_request {
+    uint32_t op;               /* ARS_OP_ENCODE or ARS_OP_RECOVER */
+    uint32_t n, k, flags;
+    uint64_t shard_len;
+    const uint8_t *positions;  /* recover: e shard indices (host; copied at submit) */
+    uint32_t e;
+    uint32_t pad;
+    const uint8_t *host_in;    /* encode: message k*L; recover: codeword n*L; copied in on
+                                  admission.  NULL: the pool slot keeps whatever it holds
+                                  (synthetic load). */
+    uint8_t *host_out;         /* encode: parity T*L; recover: codeword n*L; copied out on
+                                  completion.  NULL: no copy back. */
+} ars_request;
+
+typedef struct ars_server_stats {
+    uint64_t submitted, completed, rejected, batches;
+    uint64_t algorithmic_bytes;      /* over completed requests */
+    double p50_latency_s, p99_latency_s, max_latency_s, mean_batch_jobs;
+    double first_arrival_s, last_completion_s;  /* on the server's clock */
+} ars_server_stats;
+
+int ars_server_create(ars_ctx *ctx, const ars_server_config *cfg, ars_server **out);
+/* Stops the dispatcher after draining every submitted request. */
+int ars_server_destroy(ars_server *s);
+/* Thread-safe.  Host buffers must stay valid until the request completes.
+ * Returns ARS_OK and a ticket, or ARS_E_BUDGET if the request alone exceeds
+ * the budget (never admissible). */
+int ars_server_submit(ars_server *s, const ars_request *req, uint64_t *ticket);
+/* Blocks until the ticket completes; status is what ars_encode / ars_recover
+ * would return; latency = completion - submission, in seconds. */
+int ars_server_wait(ars_server *s, uint64_t ticket, int32_t *status, double *latency_s);
+/* Blocks until every submitted request has completed. */
+int ars_server_drain(ars_server *s);
+/* Latency percentiles over requests completed since the last reset. */
+int ars_server_stats_get(ars_server *s, ars_server_stats *out);
+int ars_server_stats_reset(ars_server *s);
+/* Load generator: submits reqs[i] at arrival_s[i] seconds after the call
+ * (from a thread of its own, so arrivals do not wait on completions), waits
+ * for all, and returns per-request statuses and latencies. */
+int ars_server_replay(ars_server *s, const ars_request *reqs, const double *arrival_s, uint32_t count,
+                      int32_t *statuses, double *latencies_s);
+
 /* Host-slice variants: same semantics on host buffers, synchronous; they
  * stage through device memory (PCIe-bound, not the HBM roofline path). */
 int ars_encode_host(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *message,

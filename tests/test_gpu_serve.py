@@ -1,0 +1,121 @@
+"""GPU tests of the serving loop: results are the single-call results whether
+a request runs alone or in a mix submitted concurrently, the HBM budget
+admits requests as memory frees up, and over-budget requests are rejected."""
+import threading
+
+import numpy as np
+import pytest
+
+import _pkg
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def ars():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return _pkg.load()
+
+
+SHAPES = [(64, 32), (128, 96), (256, 128), (256, 224), (32, 16), (256, 16)]
+
+
+def make_request_data(rng, n, k, L, op):
+    T = n - k
+    msg = rng.integers(0, 256, k * L, dtype=np.uint8)
+    if op == "encode":
+        exp = O.encode_any(n, k, msg, L) if (T & (T - 1)) == 0 else O.ext_encode_kpow2(n, k, msg, L)
+        return msg, np.zeros(T * L, np.uint8), exp, ()
+    cw = np.concatenate([O.encode_any(n, k, msg, L), msg])
+    e = int(rng.integers(1, T + 1))
+    pos = rng.permutation(n)[:e].astype(np.uint8)
+    rx = cw.copy().reshape(n, L)
+    rx[pos] = 0
+    return rx.reshape(-1).copy(), np.zeros(n * L, np.uint8), cw, pos
+
+
+def test_server_alone_equals_in_mix(ars):
+    rng = np.random.default_rng(45)
+    specs = []
+    for i in range(48):
+        n, k = SHAPES[rng.integers(len(SHAPES))]
+        L = int(64 * rng.integers(1, 64))
+        op = "encode" if rng.random() < 0.7 else "recover"
+        specs.append((op, n, k, L) + make_request_data(rng, n, k, L, op))
+    srv = ars.Server(hbm_budget=256 << 20, max_batch_jobs=16, pipeline_depth=2)
+    # alone: one request at a time
+    alone = []
+    for op, n, k, L, inp, out, exp, pos in specs:
+        o = np.zeros_like(out)
+        keep = []
+        r = ars.Server.request(op, n, k, L, pos, host_in=inp, host_out=o, keep=keep)
+        st, _ = srv.wait(srv.submit(r))
+        assert st == (0 if op == "encode" else 1)
+        assert np.array_equal(o, exp), (op, n, k, L)
+        alone.append(o)
+    # in the mix: submitted from 4 threads at once
+    outs = [np.zeros_like(sp[5]) for sp in specs]
+    keep = []
+    reqs = [ars.Server.request(sp[0], sp[1], sp[2], sp[3], sp[7], host_in=sp[4], host_out=outs[i], keep=keep)
+            for i, sp in enumerate(specs)]
+    tickets = [None] * len(reqs)
+
+    def worker(idx):
+        for i in idx:
+            tickets[i] = srv.submit(reqs[i])
+
+    th = [threading.Thread(target=worker, args=(range(t, len(reqs), 4),)) for t in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for i, tk in enumerate(tickets):
+        st, lat = srv.wait(tk)
+        assert st >= 0 and lat > 0
+        assert np.array_equal(outs[i], alone[i]), i
+    stats = srv.stats()
+    assert stats["completed"] == 2 * len(specs)
+    assert stats["batches"] < 2 * len(specs)  # the mix was batched
+    srv.close()
+
+
+def test_server_budget_admission_and_rejection(ars):
+    rng = np.random.default_rng(46)
+    n, k, L = 256, 128, 64 * 1024  # 16 MiB codewords
+    budget = 40 << 20             # two at a time
+    srv = ars.Server(hbm_budget=budget, max_batch_jobs=8, pipeline_depth=2)
+    keep, outs, exps, reqs = [], [], [], []
+    for _ in range(8):
+        inp, out, exp, pos = make_request_data(rng, n, k, L, "encode")
+        outs.append(out)
+        exps.append(exp)
+        reqs.append(ars.Server.request("encode", n, k, L, host_in=inp, host_out=out, keep=keep))
+    tickets = [srv.submit(r) for r in reqs]
+    for t, o, e in zip(tickets, outs, exps):
+        st, _ = srv.wait(t)
+        assert st == 0 and np.array_equal(o, e)
+    big = ars.Server.request("encode", 256, 128, 1 << 20)
+    with pytest.raises(ars.ArsError) as ei:
+        srv.submit(big)
+    assert ei.value.code == ars.ARS_E_BUDGET
+    srv.close()
+
+
+def test_server_replay_statuses(ars):
+    srv = ars.Server(hbm_budget=1 << 30, max_batch_jobs=64)
+    keep = []
+    reqs = (ars.Request * 6)()
+    reqs[0] = ars.Server.request("encode", 64, 32, 4096)
+    reqs[1] = ars.Server.request("recover", 64, 32, 4096, positions=list(range(33)), keep=keep)
+    reqs[2] = ars.Server.request("recover", 64, 32, 4096, positions=[5, 5], keep=keep)
+    reqs[3] = ars.Server.request("recover", 64, 32, 4096, positions=[1, 2, 3], keep=keep)
+    reqs[4] = ars.Server.request("encode", 256, 16, 1024)
+    reqs[5] = ars.Server.request("recover", 128, 96, 1024, positions=[], keep=keep)
+    st, lat = srv.replay(reqs, [0.0, 0.001, 0.001, 0.002, 0.002, 0.003])
+    assert list(st) == [0, 0, ars.ARS_E_DUPLICATE, 1, 0, 1]
+    assert (lat > 0).all()
+    srv.close()
