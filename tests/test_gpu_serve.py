@@ -91,6 +91,7 @@ def test_server_budget_admission_and_rejection(ars):
     keep, outs, exps, reqs = [], [], [], []
     for _ in range(8):
         inp, out, exp, pos = make_request_data(rng, n, k, L, "encode")
+        keep.append(inp)  # host buffers must outlive the request
         outs.append(out)
         exps.append(exp)
         reqs.append(ars.Server.request("encode", n, k, L, host_in=inp, host_out=out, keep=keep))
