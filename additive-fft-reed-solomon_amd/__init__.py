@@ -72,7 +72,16 @@ SIGNATURES = {
     "ars_server_stats_get": (_i, [_vp, _vp]),
     "ars_server_stats_reset": (_i, [_vp]),
     "ars_server_replay": (_i, [_vp, _vp, _vp, _u32, _vp, _vp]),
+    "ars_autotune": (_i, [_vp, _u32, _u32, _sz, _vp, _vp, _vp]),
+    "ars_tune_set": (_i, [_vp, _u32, _u32, _u32, _u32]),
+    "ars_batch_autotune": (_i, [_vp, _vp, _u32, ctypes.POINTER(_u32), _vp]),
 }
+
+
+class TuneResult(ctypes.Structure):
+    """struct ars_tune_result (include/ars.h)."""
+    _fields_ = [("op", ctypes.c_uint32), ("log2_size", ctypes.c_uint32), ("quads", ctypes.c_uint32),
+                ("buffers", ctypes.c_uint32), ("best_us", ctypes.c_double), ("default_us", ctypes.c_double)]
 
 ARS_OP_ENCODE = 1
 ARS_OP_RECOVER = 2
@@ -188,6 +197,17 @@ class Context:
         return _check(lib().ars_recover_with_plan(self.handle, n, k, d_received, shard_len, d_plan,
                                                   _stream_ptr(stream)))
 
+    def autotune(self, n, k, shard_len, stream=None):
+        """Time the tile variants for (n, k, shard_len) and keep the fastest."""
+        er, rr = TuneResult(), TuneResult()
+        _check(lib().ars_autotune(self.handle, n, k, shard_len, ctypes.byref(er), ctypes.byref(rr),
+                                  _stream_ptr(stream)))
+        conv = lambda r: {f: getattr(r, f) for f, _ in TuneResult._fields_}
+        return conv(er), conv(rr)
+
+    def tune_set(self, op, log2_size, quads, buffers):
+        _check(lib().ars_tune_set(self.handle, op, log2_size, quads, buffers))
+
     def recompute_data(self, n, k, d_received, shard_len, stream=None):
         return _check(lib().ars_recompute_data(self.handle, n, k, d_received, shard_len,
                                                _stream_ptr(stream)))
@@ -222,6 +242,12 @@ class Batch:
 
     def set_blocks_per_cu(self, bpc):
         _check(lib().ars_batch_set_blocks_per_cu(self.handle, bpc))
+
+    def autotune(self, arr, count, stream=None):
+        """Pick the persistent grid (workgroups per CU) for this batch shape."""
+        best = ctypes.c_uint32()
+        _check(lib().ars_batch_autotune(self.handle, arr, count, ctypes.byref(best), _stream_ptr(stream)))
+        return best.value
 
     def info(self):
         inf = BatchInfo()

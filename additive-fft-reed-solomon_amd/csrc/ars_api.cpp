@@ -123,8 +123,9 @@ int ars_encode(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *d_message, u
     const uint32_t T = n - k;
     const bool al = (L % 4 == 0) && aligned4(d_message) && aligned4(d_parity);
     if (is_pow2(T)) {
-        e = ars::launch_encode_ref(ilog2(T), al, (flags & ARS_ENC_ACCUMULATE) != 0, d_message, d_parity,
-                                   L, k / T, ctx->d_tw, s);
+        const int tl = ilog2(T);
+        e = ars::launch_encode_ref(tl, al, (flags & ARS_ENC_ACCUMULATE) != 0, d_message, d_parity,
+                                   L, k / T, ctx->d_tw, s, ctx->tune_pq[0][tl], ctx->tune_nbuf[0][tl]);
     } else {
         if (flags & ARS_ENC_ACCUMULATE)
             return fail(ARS_E_UNSUPPORTED, "ACCUMULATE is defined only when T is a power of two");
@@ -176,8 +177,9 @@ int ars_recover_with_plan(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_recei
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     const bool al = (L % 4 == 0) && aligned4(d_received);
-    e = ars::launch_recover(ilog2(n), al, d_received, L, static_cast<const ars::RecPrep *>(d_plan),
-                            ctx->d_tw, static_cast<hipStream_t>(stream));
+    const int nl = ilog2(n);
+    e = ars::launch_recover(nl, al, d_received, L, static_cast<const ars::RecPrep *>(d_plan),
+                            ctx->d_tw, static_cast<hipStream_t>(stream), ctx->tune_pq[1][nl], ctx->tune_nbuf[1][nl]);
     if (e != hipSuccess) return hip_fail(e, "recover launch");
     return ARS_TRUE;
 }
@@ -201,7 +203,9 @@ int ars_recover(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_
         return hip_fail(e, "hipMallocAsync");
     const bool al = (L % 4 == 0) && aligned4(d_received);
     e = ars::launch_recover_prep(prep, n, e_cnt, pa, ctx->d_gtab, s);
-    if (e == hipSuccess) e = ars::launch_recover(ilog2(n), al, d_received, L, prep, ctx->d_tw, s);
+    const int nl = ilog2(n);
+    if (e == hipSuccess)
+        e = ars::launch_recover(nl, al, d_received, L, prep, ctx->d_tw, s, ctx->tune_pq[1][nl], ctx->tune_nbuf[1][nl]);
     hipError_t ef = hipFreeAsync(prep, s);
     if (e != hipSuccess) return hip_fail(e, "recover launch");
     if (ef != hipSuccess) return hip_fail(ef, "hipFreeAsync");

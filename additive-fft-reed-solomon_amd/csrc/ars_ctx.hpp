@@ -12,6 +12,10 @@ struct ars_ctx {
     ars::MulTab *d_tw = nullptr;  // 255 twiddle multipliers, resident
     uint8_t *d_gtab = nullptr;    // exp / log / omega for the recover plan kernels
     int num_cus = 0;
+    // autotuned tile variant per (op, log2 transform size): 0 = default
+    // op 0 = encode (reference algorithm), 1 = recover
+    uint8_t tune_pq[2][9] = {{0}};
+    uint8_t tune_nbuf[2][9] = {{0}};
 };
 
 namespace ars {

@@ -160,41 +160,60 @@ static inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a +
 template <int TL>
 constexpr int tile_pq() { return TL >= 8 ? 32 : 64; }
 
+// Tile variants the autotuner chooses among: column quads per workgroup (PQ)
+// and LDS buffers (NBUF: 2 = one barrier per exchange, 1 = half the LDS).
+static bool variant_ok(int tl, int pq, int nbuf) {
+    if (tl <= 4) return true;
+    if (pq != 32 && pq != 64) return false;
+    if (nbuf != 1 && nbuf != 2) return false;
+    return ((1 << (tl - 4)) * pq) <= 1024;
+}
+
 template <int TL, bool AL, bool ACC>
 static hipError_t enc_one(const uint8_t *msg, uint8_t *par, uint32_t L, uint32_t chunks,
-                          const MulTab *tw, hipStream_t s) {
+                          const MulTab *tw, hipStream_t s, int pq, int nbuf) {
     const uint64_t quads = ((uint64_t)L + 3) / 4;
     if constexpr (TL <= 4) {
         hipLaunchKernelGGL((k_encode_small<TL, AL, ACC>), dim3(ceil_div(quads, 256)), dim3(256), 0, s,
                            msg, par, L, chunks, tw);
     } else {
-        constexpr int PQ = tile_pq<TL>();
-        hipLaunchKernelGGL((k_encode_tile<TL, PQ, AL, ACC, 2>), dim3(ceil_div(quads, PQ)),
-                           dim3(Tile<TL, PQ>::THREADS), 0, s, msg, par, L, chunks, tw);
+#define ENC_V(PQ_, NB_)                                                                              \
+    if (pq == PQ_ && nbuf == NB_) {                                                                  \
+        hipLaunchKernelGGL((k_encode_tile<TL, PQ_, AL, ACC, NB_>), dim3(ceil_div(quads, PQ_)),       \
+                           dim3(Tile<TL, PQ_>::THREADS), 0, s, msg, par, L, chunks, tw);             \
+        return hipGetLastError();                                                                    \
+    }
+        ENC_V(32, 1) ENC_V(32, 2) ENC_V(64, 1) ENC_V(64, 2)
+#undef ENC_V
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 template <int TL>
 static hipError_t enc_acc(bool al, bool acc, const uint8_t *msg, uint8_t *par, uint32_t L,
-                          uint32_t chunks, const MulTab *tw, hipStream_t s) {
-    if (al) return acc ? enc_one<TL, true, true>(msg, par, L, chunks, tw, s)
-                       : enc_one<TL, true, false>(msg, par, L, chunks, tw, s);
-    return acc ? enc_one<TL, false, true>(msg, par, L, chunks, tw, s)
-               : enc_one<TL, false, false>(msg, par, L, chunks, tw, s);
+                          uint32_t chunks, const MulTab *tw, hipStream_t s, int pq, int nbuf) {
+    if (al) return acc ? enc_one<TL, true, true>(msg, par, L, chunks, tw, s, pq, nbuf)
+                       : enc_one<TL, true, false>(msg, par, L, chunks, tw, s, pq, nbuf);
+    return acc ? enc_one<TL, false, true>(msg, par, L, chunks, tw, s, pq, nbuf)
+               : enc_one<TL, false, false>(msg, par, L, chunks, tw, s, pq, nbuf);
 }
 
 hipError_t launch_encode_ref(int tl, bool al, bool acc, const uint8_t *msg, uint8_t *par,
-                             uint32_t L, uint32_t chunks, const MulTab *tw, hipStream_t s) {
+                             uint32_t L, uint32_t chunks, const MulTab *tw, hipStream_t s,
+                             int pq, int nbuf) {
+    if (pq == 0) pq = tl >= 8 ? 32 : 64;
+    if (nbuf == 0) nbuf = 2;
+    if (!variant_ok(tl, pq, nbuf)) return hipErrorInvalidValue;
     switch (tl) {
-        case 0: return enc_acc<0>(al, acc, msg, par, L, chunks, tw, s);
-        case 1: return enc_acc<1>(al, acc, msg, par, L, chunks, tw, s);
-        case 2: return enc_acc<2>(al, acc, msg, par, L, chunks, tw, s);
-        case 3: return enc_acc<3>(al, acc, msg, par, L, chunks, tw, s);
-        case 4: return enc_acc<4>(al, acc, msg, par, L, chunks, tw, s);
-        case 5: return enc_acc<5>(al, acc, msg, par, L, chunks, tw, s);
-        case 6: return enc_acc<6>(al, acc, msg, par, L, chunks, tw, s);
-        case 7: return enc_acc<7>(al, acc, msg, par, L, chunks, tw, s);
+        case 0: return enc_acc<0>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
+        case 1: return enc_acc<1>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
+        case 2: return enc_acc<2>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
+        case 3: return enc_acc<3>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
+        case 4: return enc_acc<4>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
+        case 5: return enc_acc<5>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
+        case 6: return enc_acc<6>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
+        case 7: return enc_acc<7>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
         default: return hipErrorInvalidValue;
     }
 }
@@ -234,24 +253,32 @@ hipError_t launch_recover_prep(RecPrep *out, uint32_t n, uint32_t e, const PosAr
 
 template <int NL, bool AL>
 static hipError_t rec_one(uint8_t *data, uint32_t L, const RecPrep *prep, const MulTab *tw,
-                          hipStream_t s) {
+                          hipStream_t s, int pq, int nbuf) {
     const uint64_t quads = ((uint64_t)L + 3) / 4;
     if constexpr (NL <= 4) {
         hipLaunchKernelGGL((k_recover_small<NL, AL>), dim3(ceil_div(quads, 256)), dim3(256), 0, s,
                            data, L, prep, tw);
+        return hipGetLastError();
     } else {
-        constexpr int PQ = tile_pq<NL>();
-        constexpr int NBUF = NL >= 8 ? 1 : 2;
-        hipLaunchKernelGGL((k_recover_tile<NL, PQ, AL, NBUF>), dim3(ceil_div(quads, PQ)),
-                           dim3(Tile<NL, PQ>::THREADS), 0, s, data, L, prep, tw);
+#define REC_V(PQ_, NB_)                                                                              \
+    if (pq == PQ_ && nbuf == NB_) {                                                                  \
+        hipLaunchKernelGGL((k_recover_tile<NL, PQ_, AL, NB_>), dim3(ceil_div(quads, PQ_)),           \
+                           dim3(Tile<NL, PQ_>::THREADS), 0, s, data, L, prep, tw);                   \
+        return hipGetLastError();                                                                    \
     }
-    return hipGetLastError();
+        REC_V(32, 1) REC_V(32, 2) REC_V(64, 1) REC_V(64, 2)
+#undef REC_V
+        return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_recover(int nl, bool al, uint8_t *data, uint32_t L, const RecPrep *prep,
-                          const MulTab *tw, hipStream_t s) {
+                          const MulTab *tw, hipStream_t s, int pq, int nbuf) {
+    if (pq == 0) pq = nl >= 8 ? 32 : 64;
+    if (nbuf == 0) nbuf = nl >= 8 ? 1 : 2;
+    if (!variant_ok(nl, pq, nbuf)) return hipErrorInvalidValue;
 #define RC_CASE(K) \
-    case K: return al ? rec_one<K, true>(data, L, prep, tw, s) : rec_one<K, false>(data, L, prep, tw, s);
+    case K: return al ? rec_one<K, true>(data, L, prep, tw, s, pq, nbuf) : rec_one<K, false>(data, L, prep, tw, s, pq, nbuf);
     switch (nl) {
         RC_CASE(1) RC_CASE(2) RC_CASE(3) RC_CASE(4) RC_CASE(5) RC_CASE(6) RC_CASE(7) RC_CASE(8)
         default: return hipErrorInvalidValue;

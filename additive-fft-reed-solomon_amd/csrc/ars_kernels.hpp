@@ -21,14 +21,17 @@ struct PosArg {
 // exp[0..511], log[512..767], omega[768..1023].
 constexpr int kGtabBytes = 1024;
 
+// pq / nbuf select a tile variant for transforms of 32+ points (0 = default):
+// pq = column quads per workgroup (32 or 64), nbuf = LDS exchange buffers (1 or 2).
 hipError_t launch_encode_ref(int tl, bool aligned, bool accumulate, const uint8_t *msg, uint8_t *par,
-                             uint32_t L, uint32_t chunks, const MulTab *tw, hipStream_t s);
+                             uint32_t L, uint32_t chunks, const MulTab *tw, hipStream_t s,
+                             int pq = 0, int nbuf = 0);
 hipError_t launch_encode_kpow2(int kl, bool aligned, const uint8_t *msg, uint8_t *par, uint32_t L,
                                uint32_t blocks, uint32_t msg_base, const MulTab *tw, hipStream_t s);
 hipError_t launch_recover_prep(RecPrep *out, uint32_t n, uint32_t e, const PosArg &pos,
                                const uint8_t *gtab, hipStream_t s);
 hipError_t launch_recover(int nl, bool aligned, uint8_t *data, uint32_t L, const RecPrep *prep,
-                          const MulTab *tw, hipStream_t s);
+                          const MulTab *tw, hipStream_t s, int pq = 0, int nbuf = 0);
 hipError_t launch_recompute(int tl, bool aligned, uint8_t *data, uint32_t L, const MulTab *tw,
                             hipStream_t s);
 

@@ -1,0 +1,160 @@
+// ars_tune.cpp -- the autotuner half of the C ABI.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/ars.h"
+#include "ars_ctx.hpp"
+#include "ars_kernels.hpp"
+
+namespace {
+
+struct Variant {
+    int pq, nbuf;
+};
+const Variant kVariants[] = {{32, 1}, {32, 2}, {64, 1}, {64, 2}};
+constexpr int kReps = 5;
+
+// Mean time of `reps` launches of fn after one warm-up, or a negative value
+// if a launch fails.
+template <class F>
+double time_us(F fn, hipStream_t s, hipEvent_t a, hipEvent_t b) {
+    if (fn() != hipSuccess) return -1.0;
+    if (hipEventRecord(a, s) != hipSuccess) return -1.0;
+    for (int i = 0; i < kReps; ++i)
+        if (fn() != hipSuccess) return -1.0;
+    if (hipEventRecord(b, s) != hipSuccess || hipEventSynchronize(b) != hipSuccess) return -1.0;
+    float ms = 0;
+    hipEventElapsedTime(&ms, a, b);
+    return (double)ms * 1e3 / kReps;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ars_tune_set(ars_ctx *ctx, uint32_t op, uint32_t log2_size, uint32_t quads, uint32_t buffers) {
+    if (!ctx) return ars::set_error(ARS_E_NULL, "null context");
+    if ((op != ARS_OP_ENCODE && op != ARS_OP_RECOVER) || log2_size > 8 ||
+        (quads && quads != 32 && quads != 64) || (buffers && buffers != 1 && buffers != 2))
+        return ars::set_error(ARS_E_INVAL, "bad tuning entry");
+    const int o = op == ARS_OP_ENCODE ? 0 : 1;
+    ctx->tune_pq[o][log2_size] = (uint8_t)quads;
+    ctx->tune_nbuf[o][log2_size] = (uint8_t)buffers;
+    return ARS_OK;
+}
+
+int ars_autotune(ars_ctx *ctx, uint32_t n, uint32_t k, size_t L, ars_tune_result *enc_out,
+                 ars_tune_result *rec_out, void *stream) {
+    if (!ctx) return ars::set_error(ARS_E_NULL, "null context");
+    int rc = ars::check_code(n, k, L);
+    if (rc) return rc;
+    if (L % 4) return ars::set_error(ARS_E_INVAL, "autotune needs shard_len % 4 == 0");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return ars::hip_error(e, "hipSetDevice");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint32_t T = n - k;
+    uint8_t *buf = nullptr, *plan = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    if ((e = hipMalloc(&buf, (size_t)n * L)) != hipSuccess || (e = hipMalloc(&plan, sizeof(ars::RecPrep))) != hipSuccess ||
+        (e = hipEventCreate(&a)) != hipSuccess || (e = hipEventCreate(&b)) != hipSuccess) {
+        hipFree(buf);
+        hipFree(plan);
+        if (a) hipEventDestroy(a);
+        return ars::hip_error(e, "autotune scratch");
+    }
+    hipMemsetAsync(buf, 0x5a, (size_t)n * L, s);
+    // a plan for e = T erasures at every other position from 1 (the
+    // reference's test pattern, rs/src/poly_11d_lut.rs:546-555)
+    ars::PosArg pa;
+    uint32_t ecount = 0;
+    for (uint32_t p = 1; p < n && ecount < T; p += 2) pa.p[ecount++] = (uint8_t)p;
+    for (uint32_t p = 0; p < n && ecount < T; p += 2) pa.p[ecount++] = (uint8_t)p;
+    e = ars::launch_recover_prep(reinterpret_cast<ars::RecPrep *>(plan), n, ecount, pa, ctx->d_gtab, s);
+
+    for (int op = 0; op < 2 && e == hipSuccess; ++op) {
+        ars_tune_result res{};
+        res.op = op == 0 ? ARS_OP_ENCODE : ARS_OP_RECOVER;
+        const bool tunable = op == 0 ? ars::is_pow2(T) : true;
+        const int tl = op == 0 ? (tunable ? ars::ilog2(T) : ars::ilog2(k)) : ars::ilog2(n);
+        res.log2_size = (uint32_t)tl;
+        if (tunable && tl >= 5) {
+            double best = -1, dflt = -1;
+            Variant bv{0, 0};
+            for (const Variant &v : kVariants) {
+                double us;
+                if (op == 0)
+                    us = time_us([&] { return ars::launch_encode_ref(tl, true, false, buf + (size_t)T * L, buf, (uint32_t)L,
+                                                                     k / T, ctx->d_tw, s, v.pq, v.nbuf); },
+                                 s, a, b);
+                else
+                    us = time_us([&] { return ars::launch_recover(tl, true, buf, (uint32_t)L,
+                                                                  reinterpret_cast<ars::RecPrep *>(plan), ctx->d_tw, s,
+                                                                  v.pq, v.nbuf); },
+                                 s, a, b);
+                (void)hipGetLastError();
+                if (us < 0) continue;
+                const bool is_default = op == 0 ? (v.pq == (tl >= 8 ? 32 : 64) && v.nbuf == 2)
+                                                : (v.pq == (tl >= 8 ? 32 : 64) && v.nbuf == (tl >= 8 ? 1 : 2));
+                if (is_default) dflt = us;
+                if (best < 0 || us < best) {
+                    best = us;
+                    bv = v;
+                }
+            }
+            if (best > 0) {
+                ctx->tune_pq[op][tl] = (uint8_t)bv.pq;
+                ctx->tune_nbuf[op][tl] = (uint8_t)bv.nbuf;
+                res.quads = (uint32_t)bv.pq;
+                res.buffers = (uint32_t)bv.nbuf;
+                res.best_us = best;
+                res.default_us = dflt;
+            }
+        }
+        if (op == 0 && enc_out) *enc_out = res;
+        if (op == 1 && rec_out) *rec_out = res;
+    }
+    hipStreamSynchronize(s);
+    hipFree(buf);
+    hipFree(plan);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (e != hipSuccess) return ars::hip_error(e, "autotune");
+    return ARS_OK;
+}
+
+int ars_batch_autotune(ars_batch *bt, ars_job *jobs, uint32_t count, uint32_t *best_bpc, void *stream) {
+    if (!bt || !jobs) return ars::set_error(ARS_E_NULL, "null batch or jobs");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e;
+    if ((e = hipEventCreate(&a)) != hipSuccess || (e = hipEventCreate(&b)) != hipSuccess) {
+        if (a) hipEventDestroy(a);
+        return ars::hip_error(e, "hipEventCreate");
+    }
+    double best = -1;
+    uint32_t best_b = 0;
+    for (uint32_t bpc = 1; bpc <= 4; ++bpc) {
+        ars_batch_set_blocks_per_cu(bt, bpc);
+        int rc = ars_batch_run(bt, jobs, count, 0, stream);  // warm-up and graph capture
+        if (rc) break;
+        hipEventRecord(a, s);
+        for (int i = 0; i < 3; ++i) ars_batch_run(bt, jobs, count, 0, stream);
+        hipEventRecord(b, s);
+        hipEventSynchronize(b);
+        float ms = 0;
+        hipEventElapsedTime(&ms, a, b);
+        if (best < 0 || ms < best) {
+            best = ms;
+            best_b = bpc;
+        }
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (!best_b) return ars::set_error(ARS_E_STATE, "batch autotune could not run the batch");
+    ars_batch_set_blocks_per_cu(bt, best_b);
+    if (best_bpc) *best_bpc = best_b;
+    return ARS_OK;
+}
+
+}  // extern "C"

@@ -216,6 +216,30 @@ int ars_server_stats_reset(ars_server *s);
 int ars_server_replay(ars_server *s, const ars_request *reqs, const double *arrival_s, uint32_t count,
                       int32_t *statuses, double *latencies_s);
 
+/* ------------------------------------------------------------------------
+ * Autotuner.  For transforms of 32+ points the fused kernels come in tile
+ * variants: column quads per workgroup (32 or 64) and LDS exchange buffers
+ * (1: half the LDS, extra barriers; 2: one barrier per exchange).
+ * ars_autotune times every variant for one (n, k, shard_len) on scratch
+ * buffers and keeps the fastest in the context for that operation and
+ * transform size.  ars_batch_autotune picks the persistent grid (workgroups
+ * per compute unit) for a representative batch.
+ * ------------------------------------------------------------------------ */
+typedef struct ars_tune_result {
+    uint32_t op;          /* ARS_OP_ENCODE or ARS_OP_RECOVER */
+    uint32_t log2_size;   /* transform size: T for encode, n for recover */
+    uint32_t quads;       /* chosen column quads per workgroup (0: not tunable) */
+    uint32_t buffers;     /* chosen LDS exchange buffers */
+    double best_us;       /* mean kernel time of the chosen variant */
+    double default_us;    /* mean kernel time of the default variant */
+} ars_tune_result;
+
+int ars_autotune(ars_ctx *ctx, uint32_t n, uint32_t k, size_t shard_len, ars_tune_result *encode_out,
+                 ars_tune_result *recover_out, void *stream);
+int ars_tune_set(ars_ctx *ctx, uint32_t op, uint32_t log2_size, uint32_t quads, uint32_t buffers);
+int ars_batch_autotune(ars_batch *b, ars_job *jobs, uint32_t count, uint32_t *best_blocks_per_cu,
+                       void *stream);
+
 /* Host-slice variants: same semantics on host buffers, synchronous; they
  * stage through device memory (PCIe-bound, not the HBM roofline path). */
 int ars_encode_host(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *message,
