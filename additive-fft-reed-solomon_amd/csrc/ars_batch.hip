@@ -121,7 +121,7 @@ __device__ __noinline__ void run_body(const BItem &it, IO &io, int q, int p, uin
         } else if (kind == kEncKpow2) {
             if constexpr (TL <= 7) body_encode_kpow2_tile<TL, PQ, 1>(io, q, p, tile, it.arg & 0xffffu, it.arg >> 16, tw);
         } else {
-            LdsTabs tabs{s_tab, s_er};
+            LdsTabs tabs{s_tab + si * (1u << TL), s_er + si * (1u << TL)};
             body_recover_tile<TL, PQ, 1>(io, tabs, q, p, tile, tw);
         }
     }
@@ -142,7 +142,7 @@ __device__ __forceinline__ void run_tile(const BItem &it, const BSeg *segs, cons
         BufIO<true> io = make_bufio<true>(job.in, job.in_bytes, job.out, job.out_bytes,
                                           quad < job.quads ? quad : job.quads, job.L);
         run_body<TL>(it, io, q, p, tile, tw, s_tab, s_er, 0u);
-    } else if constexpr (TL <= 4) {
+    } else if constexpr (TL <= kBatchPackMaxTl) {
         BJob job;
         uint32_t col, nb, si;
         lane_job(it, segs, jobs, (uint32_t)p, PQ, r, job, col, nb, si);

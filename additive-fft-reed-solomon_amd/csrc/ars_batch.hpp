@@ -13,11 +13,13 @@ constexpr int kBatchThreads = 512;   // workgroup size of the persistent kernel
 constexpr int kBatchMaxSegs = 64;    // segments (codeword pieces) per work item
 constexpr int kBatchPlanSlots = 512; // recover: segments x n multipliers staged in LDS per item
 
-// Most segments an item of this kind / transform size may hold.  Only the
-// register-only shapes (tl <= 4) pack several codewords into one item; the
-// two-phase shapes run one codeword per item with scalar addressing.
+// Most segments an item of this kind / transform size may hold.  Shapes up
+// to 64 points pack several codewords into one item (their items are 128+
+// column quads wide, so small shards would leave most lanes idle); larger
+// shapes run one codeword per item with scalar addressing.
+constexpr int kBatchPackMaxTl = 6;
 constexpr uint32_t batch_max_segs(uint32_t kind, int tl) {
-    return tl >= 5 ? 1u
+    return tl > kBatchPackMaxTl ? 1u
                    : kind == 2u ? ((uint32_t)kBatchPlanSlots >> tl < (uint32_t)kBatchMaxSegs
                                        ? (uint32_t)kBatchPlanSlots >> tl : (uint32_t)kBatchMaxSegs)
                                 : (uint32_t)kBatchMaxSegs;
