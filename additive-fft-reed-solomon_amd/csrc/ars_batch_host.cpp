@@ -9,6 +9,7 @@
 
 #include "../../include/ars.h"
 #include "ars_batch.hpp"
+#include "ars_batch_plan.hpp"
 #include "ars_ctx.hpp"
 
 struct ars_batch {
@@ -33,29 +34,13 @@ struct ars_batch {
 
 namespace {
 
-constexpr uint32_t kRepMax = 4;  // tiles per item for large codewords
+using ars::ClassKey;
+using ars::Planned;
+
 constexpr size_t kAlign = 256;
 
 size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 bool aligned4(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; }
-
-struct ClassKey {
-    uint32_t kind, tl, acc, arg;
-    bool operator<(const ClassKey &o) const {
-        if (tl != o.tl) return tl > o.tl;  // larger transforms first: they are the long items
-        if (kind != o.kind) return kind < o.kind;
-        if (acc != o.acc) return acc < o.acc;
-        return arg < o.arg;
-    }
-    bool operator==(const ClassKey &o) const {
-        return kind == o.kind && tl == o.tl && acc == o.acc && arg == o.arg;
-    }
-};
-
-struct Planned {
-    ClassKey key;
-    uint32_t job;  // index into b->jobs
-};
 
 void destroy_graph(ars_batch *b) {
     if (b->exec) {
@@ -292,57 +277,8 @@ int ars_batch_run(ars_batch *b, ars_job *jobs, uint32_t count, uint32_t flags, v
         }
     }
 
-    // 2. work items, one code shape per item: large codewords as runs of
-    //    whole tiles, small ones packed side by side
-    std::stable_sort(planned.begin(), planned.end(),
-                     [](const Planned &x, const Planned &y) { return x.key < y.key; });
-    // (a packed item's segments must be contiguous, so the large codewords
-    //  of a class are emitted first, then its small ones are packed)
-    size_t pi = 0;
-    while (pi < planned.size()) {
-        const ClassKey key = planned[pi].key;
-        size_t pe = pi;
-        while (pe < planned.size() && planned[pe].key == key) ++pe;
-        const uint32_t lanes = ars::batch_lanes((int)key.tl);
-        const uint32_t maxseg = ars::batch_max_segs(key.kind, (int)key.tl);
-        const uint32_t meta = key.kind | (key.tl << 8) | (key.acc << 16);
-        for (size_t t = pi; t < pe; ++t) {
-            const uint32_t jid = planned[t].job;
-            const ars::BJob &bj = b->jobs[jid];
-            if (!(maxseg == 1 || bj.quads >= lanes)) continue;
-            const uint32_t tiles = (bj.quads + lanes - 1) / lanes;
-            for (uint32_t u = 0; u < tiles; u += kRepMax) {
-                const uint32_t rep = std::min(kRepMax, tiles - u);
-                b->items.push_back({meta, key.arg, (uint32_t)b->segs.size(), 1u | (rep << 16), bj.in, bj.out});
-                b->segs.push_back({jid, u * lanes, 0u, lanes});
-            }
-        }
-        int32_t open = -1;  // packed item being filled
-        uint32_t used = 0;
-        for (size_t t = pi; t < pe; ++t) {
-            const uint32_t jid = planned[t].job;
-            const ars::BJob &bj = b->jobs[jid];
-            const uint32_t quads = bj.quads;
-            if (maxseg == 1 || quads >= lanes) continue;
-            bool fits = open >= 0 && used + quads <= lanes &&
-                        (b->items[open].nseg_rep & 0xffffu) < maxseg;
-            if (fits) {
-                const ars::BItem &it = b->items[open];
-                fits = bj.in >= it.in_base && bj.out >= it.out_base &&
-                       (uint64_t)(bj.in - it.in_base) + bj.in_bytes <= 0xffffffffull &&
-                       (uint64_t)(bj.out - it.out_base) + bj.out_bytes <= 0xffffffffull;
-            }
-            if (!fits) {
-                open = (int32_t)b->items.size();
-                used = 0;
-                b->items.push_back({meta, key.arg, (uint32_t)b->segs.size(), 0u | (1u << 16), bj.in, bj.out});
-            }
-            b->segs.push_back({jid, 0u, used, quads});
-            b->items[open].nseg_rep += 1u;
-            used += quads;
-        }
-        pi = pe;
-    }
+    // 2. work items (ars_batch_plan.hpp)
+    ars::plan_items(planned, b->jobs, b->items, b->segs);
 
     // 3. arena layout: header | items | segs | jobs | preps | positions | plans
     ars::BHeader hdr{};
