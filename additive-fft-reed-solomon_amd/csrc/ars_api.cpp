@@ -105,6 +105,7 @@ int ars_create(int device, ars_ctx **out) {
 int ars_destroy(ars_ctx *ctx) {
     if (!ctx) return ARS_OK;
     hipSetDevice(ctx->device);
+    ars::release_host_pipe(ctx);
     hipFree(ctx->d_tw);
     hipFree(ctx->d_gtab);
     delete ctx;
@@ -224,53 +225,6 @@ int ars_recompute_data(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received
     e = ars::launch_recompute(ilog2(n - k), al, d_received, L, ctx->d_tw, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "recompute launch");
     return ARS_OK;
-}
-
-int ars_encode_host(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *message, uint8_t *parity,
-                    size_t L, uint32_t flags) {
-    if (!ctx || !message || !parity) return fail(ARS_E_NULL, "null context or buffer");
-    int rc = check_code(n, k, L);
-    if (rc) return rc;
-    hipError_t e = hipSetDevice(ctx->device);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    const size_t mb = (size_t)k * L, pb = (size_t)(n - k) * L;
-    uint8_t *dm = nullptr, *dp = nullptr;
-    if ((e = hipMalloc(&dm, mb)) != hipSuccess || (e = hipMalloc(&dp, pb)) != hipSuccess) {
-        hipFree(dm);
-        return hip_fail(e, "hipMalloc");
-    }
-    e = hipMemcpy(dm, message, mb, hipMemcpyHostToDevice);
-    if (e == hipSuccess && (flags & ARS_ENC_ACCUMULATE)) e = hipMemcpy(dp, parity, pb, hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-        rc = ars_encode(ctx, n, k, dm, dp, L, flags, nullptr);
-        if (rc == ARS_OK) e = hipMemcpy(parity, dp, pb, hipMemcpyDeviceToHost);
-    }
-    hipFree(dm);
-    hipFree(dp);
-    if (e != hipSuccess) return hip_fail(e, "encode_host");
-    return rc;
-}
-
-int ars_recover_host(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *received, size_t L,
-                     const uint8_t *positions, uint32_t e_cnt) {
-    if (!ctx || !received) return fail(ARS_E_NULL, "null context or buffer");
-    int rc = check_code(n, k, L);
-    if (rc) return rc;
-    if (e_cnt > n - k) return ARS_FALSE;
-    if (e_cnt == 0) return ARS_TRUE;
-    hipError_t e = hipSetDevice(ctx->device);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    const size_t b = (size_t)n * L;
-    uint8_t *d = nullptr;
-    if ((e = hipMalloc(&d, b)) != hipSuccess) return hip_fail(e, "hipMalloc");
-    e = hipMemcpy(d, received, b, hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-        rc = ars_recover(ctx, n, k, d, L, positions, e_cnt, nullptr);
-        if (rc == ARS_TRUE) e = hipMemcpy(received, d, b, hipMemcpyDeviceToHost);
-    }
-    hipFree(d);
-    if (e != hipSuccess) return hip_fail(e, "recover_host");
-    return rc;
 }
 
 }  // extern "C"

@@ -24,4 +24,5 @@ int hip_error(hipError_t e, const char *where);
 int check_code(uint32_t n, uint32_t k, size_t L);
 bool is_pow2(uint32_t v);
 int ilog2(uint32_t v);
+void release_host_pipe(ars_ctx *ctx);  // ars_host.cpp
 }  // namespace ars

@@ -166,6 +166,31 @@ def test_recompute_data_matches_oracle(ars):
         assert np.array_equal(host(d), cw), n
 
 
+@pytest.mark.parametrize("n,k,L", [(256, 128, 200 * 1024 + 64), (64, 32, 70001), (256, 16, 150000)])
+def test_host_slice_streaming_chunks(ars, n, k, L):
+    """Host slices larger than one pipeline chunk (and unaligned lengths)."""
+    rng = np.random.default_rng(L)
+    T = n - k
+    msg = rng.integers(0, 256, k * L, dtype=np.uint8)
+    par = np.zeros(T * L, np.uint8)
+    c = ars.Codec(n, k)
+    c.encode_systematic_sharded(msg, par, None, L, accumulate=False)
+    exp = O.encode_any(n, k, msg, L)
+    assert np.array_equal(par, exp)
+    if T & (T - 1) == 0:  # accumulate into non-zero parity, chunk by chunk
+        par0 = rng.integers(0, 256, T * L, dtype=np.uint8)
+        acc = par0.copy()
+        c.encode_systematic_sharded(msg, acc, None, L)
+        assert np.array_equal(acc, O.encode(n, T, msg, L, parity=par0.copy(), gfni=O.gfni_supported()))
+    cw = np.concatenate([par, msg])
+    pos = rng.permutation(n)[:T].astype(np.uint8)
+    rx = cw.copy().reshape(n, L)
+    rx[pos] = 0
+    rx = rx.reshape(-1).copy()
+    assert c.recover_erasure_shards(rx, None, L, pos)
+    assert np.array_equal(rx, cw)
+
+
 def test_host_slice_variants(ars):
     rng = np.random.default_rng(3)
     n, k, L = 128, 96, 256
