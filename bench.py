@@ -201,6 +201,57 @@ def bench_served(ars, np, rates=(1000, 2000, 4000, 8000, 12000), per_rate=1500, 
             "sweep": out}
 
 
+def bench_host(ars, torch, np, n, k, L, e, reps=5, seed=46):
+    """The host-slice entry points (the Rust binding's path): the same
+    (n, k, L) codeword in pageable host memory, streamed through the GPU in
+    column chunks.  PCIe-inclusive wall-clock rate, beside the pinned-copy
+    rate of the link in each direction for scale."""
+    T = n - k
+    rng = np.random.default_rng(seed)
+    msg = rng.integers(0, 256, k * L, dtype=np.uint8)
+    par = np.zeros(T * L, np.uint8)
+    c = ars.Codec(n, k)
+    c.encode_systematic_sharded(msg, par, None, L, accumulate=False)  # warm-up (pipeline set-up)
+    cw = np.concatenate([par, msg])
+    pos = rng.permutation(n)[:e].astype(np.uint8)
+    t_enc = t_rec = 0.0
+    for _ in range(reps):
+        t = time.perf_counter()
+        c.encode_systematic_sharded(msg, par, None, L, accumulate=False)
+        t_enc += time.perf_counter() - t
+        rx = cw.copy().reshape(n, L)
+        rx[pos] = 0  # the reference's contract: erased shards arrive zeroed
+        rx = rx.reshape(-1)
+        t = time.perf_counter()
+        ok = c.recover_erasure_shards(rx, None, L, pos)
+        t_rec += time.perf_counter() - t
+        assert ok
+    assert np.array_equal(rx, cw)
+    enc_b, rec_b = n * L, (n + e) * L
+    big = 256 << 20
+    h = torch.empty(big, dtype=torch.uint8, pin_memory=True)
+    d = torch.empty(big, dtype=torch.uint8, device="cuda")
+    d.copy_(h, non_blocking=True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(3):
+        d.copy_(h, non_blocking=True)
+    torch.cuda.synchronize()
+    h2d = 3 * big / (time.perf_counter() - t) / 1e9
+    t = time.perf_counter()
+    for _ in range(3):
+        h.copy_(d, non_blocking=True)
+    torch.cuda.synchronize()
+    d2h = 3 * big / (time.perf_counter() - t) / 1e9
+    del h, d
+    return {"workload": f"host slices (n={n},k={k}), L={L}, e={e}, pageable numpy buffers",
+            "encode_GBps": round(enc_b * reps / t_enc / 1e9, 2),
+            "recover_GBps": round(rec_b * reps / t_rec / 1e9, 2),
+            "encode_ms": round(t_enc / reps * 1e3, 2), "recover_ms": round(t_rec / reps * 1e3, 2),
+            "pcie_bytes_encode": n * L, "pcie_bytes_recover": (n + e) * L,
+            "pinned_h2d_GBps": round(h2d, 1), "pinned_d2h_GBps": round(d2h, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +261,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-host", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every figure is "
@@ -343,6 +395,9 @@ def main():
         out["batched"] = bench_batched(ars, torch, np, stream)
     if not args.no_served:
         out["served"] = bench_served(ars, np)
+    if not args.no_host:
+        out["host_slices"] = bench_host(ars, torch, np, n, k, L, e)
+        print(f"host slices: {out['host_slices']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, k, L, e, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -166,9 +166,11 @@ def test_recompute_data_matches_oracle(ars):
         assert np.array_equal(host(d), cw), n
 
 
-@pytest.mark.parametrize("n,k,L", [(256, 128, 200 * 1024 + 64), (64, 32, 70001), (256, 16, 150000)])
-def test_host_slice_streaming_chunks(ars, n, k, L):
-    """Host slices larger than one pipeline chunk (and unaligned lengths)."""
+@pytest.mark.parametrize("n,k,L,e", [(256, 128, 200 * 1024 + 64, 128), (64, 32, 70001, 32), (256, 16, 150000, 240),
+                                     (256, 128, 131072 + 1001, 3), (128, 64, 262144 + 4097, 1)])
+def test_host_slice_streaming_chunks(ars, n, k, L, e):
+    """Host slices larger than one pipeline chunk, ragged last chunks, few or
+    many erasures."""
     rng = np.random.default_rng(L)
     T = n - k
     msg = rng.integers(0, 256, k * L, dtype=np.uint8)
@@ -183,7 +185,7 @@ def test_host_slice_streaming_chunks(ars, n, k, L):
         c.encode_systematic_sharded(msg, acc, None, L)
         assert np.array_equal(acc, O.encode(n, T, msg, L, parity=par0.copy(), gfni=O.gfni_supported()))
     cw = np.concatenate([par, msg])
-    pos = rng.permutation(n)[:T].astype(np.uint8)
+    pos = rng.permutation(n)[:e].astype(np.uint8)
     rx = cw.copy().reshape(n, L)
     rx[pos] = 0
     rx = rx.reshape(-1).copy()
