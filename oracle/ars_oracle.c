@@ -393,3 +393,422 @@ int aro_ext_recover_lagrange(size_t n, size_t k, uint8_t *received, size_t L,
     }
     return 1;
 }
+
+/* ---------------------------------------------------------------------------
+ * Scalar error decoding: Codec::decode_systematic_scalar
+ * (rs/src/codec.rs:341-420) with its syndrome (rs/src/codec.rs:266-301), its
+ * key-equation solvers (HGCD rs/src/codec.rs:84-210, EEA rs/src/codec.rs:28-80)
+ * and the novel-basis polynomial arithmetic they call
+ * (rs/src/poly_arith.rs:56-116, 198-319).
+ *
+ * A polynomial is an array of NMAX coefficients in the basis X_j, of which the
+ * first n are meaningful (the reference's [G; N]).  Where the reference would
+ * panic (a slice or shift beyond N, a shift by a negative amount in its debug
+ * build), the restatement returns ARO_PANIC up the call chain.
+ */
+#define NMAX 256
+#define ARO_PANIC (-3)
+
+static int pdeg(const uint8_t *p, size_t n) {
+    for (size_t i = n; i-- > 0;)
+        if (p[i]) return (int)i;
+    return -1;
+}
+
+static void padd(uint8_t *dst, const uint8_t *a, const uint8_t *b, size_t n) {
+    for (size_t i = 0; i < n; i++) dst[i] = a[i] ^ b[i];
+}
+
+/* rs/src/poly_arith.rs:198-231: product by FFT of the smallest power-of-two
+ * size above deg a + deg b. */
+static int poly_mul_lnh(const uint8_t *a, const uint8_t *b, size_t n, uint8_t *out) {
+    int da = pdeg(a, n), db = pdeg(b, n);
+    if (da < 0 || db < 0) {
+        memset(out, 0, n);
+        return 0;
+    }
+    size_t m = 1;
+    while (m < (size_t)(da + db + 1)) m <<= 1;
+    if (m > n) return ARO_PANIC; /* va[..m] with m > N */
+    uint8_t va[NMAX], vb[NMAX];
+    memcpy(va, a, n);
+    memcpy(vb, b, n);
+    int k = ilog2(m);
+    aro_fft_scalar(va, k, 0);
+    aro_fft_scalar(vb, k, 0);
+    for (size_t i = 0; i < m; i++) va[i] = aro_mul(va[i], vb[i]);
+    aro_ifft_scalar(va, k, 0);
+    memcpy(out, va, n);
+    return 0;
+}
+
+/* rs/src/poly_arith.rs:233-267: long division, one basis-aware product per
+ * quotient term.  Returns 1 (Some), 0 (None: b = 0) or ARO_PANIC. */
+static int poly_div_lnh(const uint8_t *a, const uint8_t *b, size_t n, uint8_t *q, uint8_t *r) {
+    int bd = pdeg(b, n);
+    if (bd < 0) return 0;
+    uint8_t rr[NMAX], qq[NMAX], term[NMAX], prod[NMAX];
+    memcpy(rr, a, n);
+    memset(qq, 0, n);
+    const uint8_t binv = INV[b[bd]];
+    for (;;) {
+        int rd = pdeg(rr, n);
+        if (rd < bd) break; /* also rd < 0 */
+        int diff = rd - bd;
+        uint8_t f = aro_mul(rr[rd], binv);
+        qq[diff] ^= f;
+        memset(term, 0, n);
+        term[diff] = f;
+        if (poly_mul_lnh(term, b, n, prod)) return ARO_PANIC;
+        for (size_t i = 0; i < n; i++) rr[i] ^= prod[i];
+    }
+    memcpy(q, qq, n);
+    memcpy(r, rr, n);
+    return 1;
+}
+
+/* rs/src/poly_arith.rs:91-116 (poly::deriv_lnh), over all N coefficients. */
+static void deriv_lnh(const uint8_t *c, size_t n, uint8_t *out) {
+    memset(out, 0, n);
+    if (n <= 1) return;
+    memcpy(out, c, n);
+    int m = ilog2(n);
+    for (int j = 1; j <= m; j++) {
+        size_t half = (size_t)1 << (j - 1), step = (size_t)1 << j;
+        for (size_t s = 0; s < n; s += step)
+            for (size_t i = 0; i < half; i++) out[s + i] ^= out[s + i + half];
+    }
+}
+
+/* rs/src/poly_arith.rs:69-89 (poly::split_at, poly::shift_up). */
+static int split_at(const uint8_t *p, int k, size_t n, uint8_t *lo, uint8_t *hi) {
+    size_t pivot = (size_t)1 << k;
+    if (pivot > n) return ARO_PANIC;
+    uint8_t l[NMAX], h[NMAX];
+    memset(l, 0, n);
+    memset(h, 0, n);
+    memcpy(l, p, pivot);
+    memcpy(h, p + pivot, n - pivot);
+    memcpy(lo, l, n);
+    memcpy(hi, h, n);
+    return 0;
+}
+static int shift_up(const uint8_t *p, int k, size_t n, uint8_t *out) {
+    size_t shift = (size_t)1 << k;
+    if (shift > n) return ARO_PANIC;
+    uint8_t o[NMAX];
+    memset(o, 0, n);
+    memcpy(o + shift, p, n - shift);
+    memcpy(out, o, n);
+    return 0;
+}
+
+typedef struct {
+    uint8_t m[4][NMAX]; /* row-major [m00, m01, m10, m11] */
+} pmat;
+
+/* rs/src/poly_arith.rs:292-319 (mat_vec_lnh, mat_mul_lnh). */
+static int mat_vec(const pmat *M, const uint8_t *v0, const uint8_t *v1, size_t n, uint8_t *o0, uint8_t *o1) {
+    uint8_t t0[NMAX], t1[NMAX], t2[NMAX], t3[NMAX];
+    if (poly_mul_lnh(M->m[0], v0, n, t0) || poly_mul_lnh(M->m[1], v1, n, t1) ||
+        poly_mul_lnh(M->m[2], v0, n, t2) || poly_mul_lnh(M->m[3], v1, n, t3))
+        return ARO_PANIC;
+    padd(o0, t0, t1, n);
+    padd(o1, t2, t3, n);
+    return 0;
+}
+static int mat_mul(const pmat *A, const pmat *B, size_t n, pmat *out) {
+    pmat r;
+    uint8_t t0[NMAX], t1[NMAX];
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) {
+            if (poly_mul_lnh(A->m[2 * i], B->m[j], n, t0) || poly_mul_lnh(A->m[2 * i + 1], B->m[2 + j], n, t1))
+                return ARO_PANIC;
+            padd(r.m[2 * i + j], t0, t1, n);
+        }
+    *out = r;
+    return 0;
+}
+
+static void mat_identity(pmat *M, size_t n) {
+    for (int i = 0; i < 4; i++) memset(M->m[i], 0, n);
+    M->m[0][0] = 1;
+    M->m[3][0] = 1;
+}
+
+/* rs/src/codec.rs:84-122 (poly_hgcd_middle). */
+static int hgcd_middle(const uint8_t *p, int g, size_t n, uint8_t *p_ll, uint8_t *p_m) {
+    if (g < 2) return ARO_PANIC; /* 1 << (g - 2) underflows */
+    size_t q = (size_t)1 << (g - 2), h = (size_t)1 << (g - 1);
+    if (q + 2 * h > n) return ARO_PANIC;
+    const uint8_t c = SPV[g - 2][SUBSPACE[q]];
+    uint8_t ll[NMAX], mm[NMAX];
+    memset(ll, 0, n);
+    memset(mm, 0, n);
+    memcpy(ll, p, q);
+    for (size_t i = 0; i < q; i++) {
+        mm[i] = p[i + q] ^ aro_mul_lut(c, p[i + h]);
+        mm[i + q] = p[i + h];
+        mm[i + h] = p[i + h + q] ^ aro_mul_lut(c, p[i + 2 * h]);
+        mm[i + h + q] = p[i + 2 * h];
+    }
+    memcpy(p_ll, ll, n);
+    memcpy(p_m, mm, n);
+    return 0;
+}
+
+/* rs/src/codec.rs:124-210 (hgcd, LNH Algorithm 5). */
+static int hgcd(const uint8_t *a, const uint8_t *b, int g, size_t n, uint8_t *z0, uint8_t *z1, pmat *M) {
+    if (g < 1) return ARO_PANIC; /* 1 << (g - 1) underflows */
+    const int half = 1 << (g - 1);
+    int bd = pdeg(b, n);
+    if (bd < half) { /* lines 1-2 */
+        memcpy(z0, a, n);
+        memcpy(z1, b, n);
+        mat_identity(M, n);
+        return 0;
+    }
+    uint8_t a_l[NMAX], a_h[NMAX], b_l[NMAX], b_h[NMAX];
+    if (split_at(a, g - 1, n, a_l, a_h) || split_at(b, g - 1, n, b_l, b_h)) return ARO_PANIC;
+    uint8_t zh0[NMAX], zh1[NMAX];
+    pmat mh;
+    int rc = hgcd(a_h, b_h, g - 1, n, zh0, zh1, &mh);
+    if (rc) return rc;
+    uint8_t mv0[NMAX], mv1[NMAX], zm0[NMAX], zm1[NMAX], t[NMAX];
+    if (mat_vec(&mh, a_l, b_l, n, mv0, mv1)) return ARO_PANIC;
+    if (shift_up(zh0, g - 1, n, t)) return ARO_PANIC;
+    padd(zm0, t, mv0, n);
+    if (shift_up(zh1, g - 1, n, t)) return ARO_PANIC;
+    padd(zm1, t, mv1, n);
+    if (pdeg(zm1, n) < half) { /* lines 5-6 */
+        memcpy(z0, zm0, n);
+        memcpy(z1, zm1, n);
+        *M = mh;
+        return 0;
+    }
+    uint8_t qm[NMAX], rm[NMAX];
+    rc = poly_div_lnh(zm0, zm1, n, qm, rm);
+    if (rc != 1) return ARO_PANIC; /* .expect() */
+    uint8_t zm1_ll[NMAX], zm1_m[NMAX], rm_ll[NMAX], rm_m[NMAX];
+    if (hgcd_middle(zm1, g, n, zm1_ll, zm1_m) || hgcd_middle(rm, g, n, rm_ll, rm_m)) return ARO_PANIC;
+    uint8_t ym0[NMAX], ym1[NMAX];
+    pmat mm;
+    rc = hgcd(zm1_m, rm_m, g - 1, n, ym0, ym1, &mm);
+    if (rc) return rc;
+    pmat swap, tmp, mr;
+    for (int i = 0; i < 4; i++) memset(swap.m[i], 0, n);
+    swap.m[1][0] = 1;
+    swap.m[2][0] = 1;
+    memcpy(swap.m[3], qm, n);
+    if (mat_mul(&swap, &mh, n, &tmp) || mat_mul(&mm, &tmp, n, &mr)) return ARO_PANIC;
+    uint8_t sg2[NMAX];
+    memset(sg2, 0, n);
+    if (((size_t)1 << (g - 2)) >= n) return ARO_PANIC;
+    sg2[(size_t)1 << (g - 2)] = 1;
+    if (mat_vec(&mm, zm1_ll, rm_ll, n, mv0, mv1)) return ARO_PANIC;
+    uint8_t zr0[NMAX], zr1[NMAX];
+    if (poly_mul_lnh(ym0, sg2, n, t)) return ARO_PANIC;
+    padd(zr0, t, mv0, n);
+    if (shift_up(ym1, g - 2, n, t)) return ARO_PANIC;
+    padd(zr1, t, mv1, n);
+    memcpy(z0, zr0, n);
+    memcpy(z1, zr1, n);
+    *M = mr;
+    return 0;
+}
+
+/* rs/src/codec.rs:212-239 (solve_key_equation_hgcd).  Returns 1 (Some),
+ * 0 (None) or ARO_PANIC; v1 = m11, lambda = m10 + m11 q_t. */
+static int key_equation_hgcd(const uint8_t *syn, size_t n, size_t T, uint8_t *v1, uint8_t *lambda) {
+    uint8_t st[NMAX], qt[NMAX], rt[NMAX], z0[NMAX], z1[NMAX], t[NMAX];
+    memset(st, 0, n);
+    st[T] = 1;
+    int rc = poly_div_lnh(st, syn, n, qt, rt);
+    if (rc != 1) return rc;
+    pmat M;
+    rc = hgcd(syn, rt, ilog2(T), n, z0, z1, &M);
+    if (rc) return rc;
+    memcpy(v1, M.m[3], n);
+    if (poly_mul_lnh(M.m[3], qt, n, t)) return ARO_PANIC;
+    padd(lambda, M.m[2], t, n);
+    return 1;
+}
+
+/* rs/src/codec.rs:28-80 (solve_key_equation_eea and eea). */
+static int key_equation_eea(const uint8_t *syn, size_t n, size_t T, uint8_t *v1_out, uint8_t *lambda) {
+    uint8_t st[NMAX], qt[NMAX], rt[NMAX];
+    memset(st, 0, n);
+    st[T] = 1;
+    int rc = poly_div_lnh(st, syn, n, qt, rt);
+    if (rc != 1) return rc;
+    uint8_t z0[NMAX], z1[NMAX], u0[NMAX], u1[NMAX], v0[NMAX], v1[NMAX];
+    uint8_t q[NMAX], rem[NMAX], t[NMAX], nx[NMAX];
+    memcpy(z0, syn, n);
+    memcpy(z1, rt, n);
+    memset(u0, 0, n);
+    memset(u1, 0, n);
+    memset(v0, 0, n);
+    memset(v1, 0, n);
+    u0[0] = 1;
+    v1[0] = 1;
+    const int target = (int)(T / 2);
+    while (pdeg(z1, n) >= target) {
+        if (poly_div_lnh(z0, z1, n, q, rem) != 1) return ARO_PANIC;
+        memcpy(z0, z1, n);
+        memcpy(z1, rem, n);
+        if (poly_mul_lnh(q, u1, n, t)) return ARO_PANIC;
+        padd(nx, u0, t, n);
+        memcpy(u0, u1, n);
+        memcpy(u1, nx, n);
+        if (poly_mul_lnh(q, v1, n, t)) return ARO_PANIC;
+        padd(nx, v0, t, n);
+        memcpy(v0, v1, n);
+        memcpy(v1, nx, n);
+    }
+    memcpy(v1_out, v1, n);
+    if (poly_mul_lnh(v1, qt, n, t)) return ARO_PANIC;
+    padd(lambda, u1, t, n);
+    return 1;
+}
+
+/* rs/src/codec.rs:266-301: s = sum_i IFFT_T(chunk i, omega_{iT}), N entries
+ * of which the first T are set. */
+static void syndrome_scalar(const uint8_t *r, size_t n, size_t T, uint8_t *syn) {
+    memset(syn, 0, n);
+    uint8_t ws[NMAX];
+    for (size_t i = 0; i < n / T; i++) {
+        memcpy(ws, r + i * T, T);
+        aro_ifft_scalar(ws, ilog2(T), SUBSPACE[i * T]);
+        for (size_t j = 0; j < T; j++) syn[j] ^= ws[j];
+    }
+}
+
+/* The formal derivative in the basis X (an extension, not the reference's
+ * operator): every s_b has x-coefficient 1 (gf2p8/src/poly_11d.rs:204-213),
+ * so X_j' = sum_{b in j} X_{j - 2^b} and D'[i] = xor_{b not in i} D[i | 2^b]. */
+static void formal_deriv(const uint8_t *c, size_t n, uint8_t *out) {
+    const int nb = ilog2(n);
+    for (size_t i = 0; i < n; i++) {
+        uint8_t v = 0;
+        for (int b = 0; b < nb; b++)
+            if (!((i >> b) & 1)) v ^= c[i | ((size_t)1 << b)];
+        out[i] = v;
+    }
+}
+
+/* rs/src/codec.rs:341-420.  Returns 1 (true), 0 (false), -1 for a code
+ * outside the reference's contract, or ARO_PANIC.  *kind: 0 zero syndrome,
+ * 1 true after the key equation, 2 false, 3 panic.  formal != 0 replaces the
+ * reference's deriv_lnh by the formal derivative in step 4 (extension). */
+int aro_decode_systematic_scalar(size_t n, size_t T, uint8_t *received, int *kind, int formal) {
+    build_tables();
+    int dummy;
+    if (!kind) kind = &dummy;
+    if (!valid_ref_code(n, T)) return -1;
+    const int t_log = ilog2(T);
+    uint8_t syn[NMAX];
+    syndrome_scalar(received, n, T, syn);
+    int zero = 1;
+    for (size_t j = 0; j < T; j++) zero &= syn[j] == 0;
+    if (zero) {
+        *kind = 0;
+        return 1;
+    }
+    uint8_t v1[NMAX], lambda[NMAX];
+    int rc = key_equation_hgcd(syn, n, T, v1, lambda);
+    if (rc == ARO_PANIC) {
+        *kind = 3;
+        return ARO_PANIC;
+    }
+    if (rc == 0) {
+        *kind = 2;
+        return 0;
+    }
+    *kind = 1;
+    int dl = pdeg(lambda, n);
+    if (dl < 0) return 1; /* zero locator */
+    /* step 3: root search, one T-point FFT per chunk, first dl roots */
+    size_t idx[NMAX];
+    int cnt = 0;
+    uint8_t ev[NMAX];
+    for (size_t c = 0; c < n / T && cnt < dl; c++) {
+        memcpy(ev, lambda, n);
+        aro_fft_scalar(ev, t_log, SUBSPACE[c * T]);
+        for (size_t off = 0; off < T; off++)
+            if (ev[off] == 0) {
+                idx[cnt++] = c * T + off;
+                if (cnt == dl) break;
+            }
+    }
+    if (cnt < dl) {
+        *kind = 2;
+        return 0;
+    }
+    /* step 4: error values q(w_p) / lambda'(w_p), lambda' = deriv_lnh */
+    uint8_t lp[NMAX], qe[NMAX], le[NMAX];
+    if (formal)
+        formal_deriv(lambda, n, lp);
+    else
+        deriv_lnh(lambda, n, lp);
+    for (size_t c = 0; c < n / T; c++) {
+        int any = 0;
+        for (int i = 0; i < cnt; i++) any |= idx[i] / T == c;
+        if (!any) continue;
+        memcpy(qe, v1, n);
+        memcpy(le, lp, n);
+        aro_fft_scalar(qe, t_log, SUBSPACE[c * T]);
+        aro_fft_scalar(le, t_log, SUBSPACE[c * T]);
+        for (int i = 0; i < cnt; i++) {
+            if (idx[i] / T != c) continue;
+            size_t off = idx[i] % T;
+            if (le[off] == 0) continue;
+            received[idx[i]] ^= aro_mul_lut(qe[off], INV[le[off]]);
+        }
+    }
+    return 1;
+}
+
+/* Sharded form (no reference counterpart; README.md:58 lists sharded error
+ * correction as not implemented): byte column j of the n x L codeword is the
+ * received word decode_systematic_scalar sees.  status[j] = its *kind.
+ * Returns 1 when every column returned true, 0 when one returned false,
+ * ARO_PANIC when one would have panicked, -1 for a bad code. */
+int aro_decode_sharded(size_t n, size_t T, uint8_t *received, size_t L, uint8_t *status, int formal) {
+    build_tables();
+    if (!valid_ref_code(n, T)) return -1;
+    int all = 1, panic = 0;
+    uint8_t col[NMAX];
+    for (size_t j = 0; j < L; j++) {
+        for (size_t i = 0; i < n; i++) col[i] = received[i * L + j];
+        int kind;
+        int rc = aro_decode_systematic_scalar(n, T, col, &kind, formal);
+        if (status) status[j] = (uint8_t)kind;
+        if (rc == 1)
+            for (size_t i = 0; i < n; i++) received[i * L + j] = col[i];
+        else if (rc == 0)
+            all = 0;
+        else
+            panic = 1;
+    }
+    return panic ? ARO_PANIC : all;
+}
+
+/* Exported pieces, for pinning each step on its own. */
+int aro_poly_mul_lnh(const uint8_t *a, const uint8_t *b, size_t n, uint8_t *out) {
+    build_tables();
+    return poly_mul_lnh(a, b, n, out);
+}
+int aro_poly_div_lnh(const uint8_t *a, const uint8_t *b, size_t n, uint8_t *q, uint8_t *r) {
+    build_tables();
+    return poly_div_lnh(a, b, n, q, r);
+}
+void aro_deriv_lnh(const uint8_t *c, size_t n, uint8_t *out) { deriv_lnh(c, n, out); }
+void aro_formal_deriv(const uint8_t *c, size_t n, uint8_t *out) { formal_deriv(c, n, out); }
+void aro_syndrome_scalar(const uint8_t *r, size_t n, size_t T, uint8_t *syn) {
+    build_tables();
+    syndrome_scalar(r, n, T, syn);
+}
+int aro_key_equation(const uint8_t *syn, size_t n, size_t T, int use_eea, uint8_t *v1, uint8_t *lambda) {
+    build_tables();
+    return use_eea ? key_equation_eea(syn, n, T, v1, lambda) : key_equation_hgcd(syn, n, T, v1, lambda);
+}

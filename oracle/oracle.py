@@ -61,6 +61,22 @@ def lib():
         L.aro_ext_recover_lagrange.argtypes = [_sz, _sz, _u8p, _sz, _u8p, _sz]
         L.aro_gfni_supported.restype = ctypes.c_int
         L.aro_gfni_supported.argtypes = []
+        L.aro_decode_systematic_scalar.restype = ctypes.c_int
+        L.aro_decode_systematic_scalar.argtypes = [_sz, _sz, _u8p, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        L.aro_decode_sharded.restype = ctypes.c_int
+        L.aro_decode_sharded.argtypes = [_sz, _sz, _u8p, _sz, _u8p, ctypes.c_int]
+        L.aro_formal_deriv.restype = None
+        L.aro_formal_deriv.argtypes = [_u8p, _sz, _u8p]
+        L.aro_poly_mul_lnh.restype = ctypes.c_int
+        L.aro_poly_mul_lnh.argtypes = [_u8p, _u8p, _sz, _u8p]
+        L.aro_poly_div_lnh.restype = ctypes.c_int
+        L.aro_poly_div_lnh.argtypes = [_u8p, _u8p, _sz, _u8p, _u8p]
+        L.aro_deriv_lnh.restype = None
+        L.aro_deriv_lnh.argtypes = [_u8p, _sz, _u8p]
+        L.aro_syndrome_scalar.restype = None
+        L.aro_syndrome_scalar.argtypes = [_u8p, _sz, _sz, _u8p]
+        L.aro_key_equation.restype = ctypes.c_int
+        L.aro_key_equation.argtypes = [_u8p, _sz, _sz, ctypes.c_int, _u8p, _u8p]
         _lib = L
     return _lib
 
@@ -168,3 +184,92 @@ def recover_any(n, k, received, L, positions):
     if T & (T - 1) == 0:
         return recover(n, T, received, L, positions)
     return ext_recover_lagrange(n, k, received, L, positions)
+
+
+# --- error decoding (rs/src/codec.rs:28-301, 341-420; rs/src/poly_arith.rs) ---
+
+PANIC = -3  # the reference would panic at this input
+
+
+def _poly(a, n):
+    out = np.zeros(n, np.uint8)
+    a = np.asarray(a, dtype=np.uint8)
+    out[:len(a)] = a
+    return out
+
+
+def poly_mul_lnh(a, b, n=256):
+    """CantorBasisPolyArith::poly_mul_lnh (rs/src/poly_arith.rs:198-231) on
+    [G; n] arrays.  Returns the product, or None where the reference panics."""
+    a, b, out = _poly(a, n), _poly(b, n), np.zeros(n, np.uint8)
+    rc = lib().aro_poly_mul_lnh(_p(a), _p(b), n, _p(out))
+    return None if rc else out
+
+
+def poly_div_lnh(a, b, n=256):
+    """poly_div_lnh (rs/src/poly_arith.rs:233-267): (q, r), or None for b = 0."""
+    a, b = _poly(a, n), _poly(b, n)
+    q, r = np.zeros(n, np.uint8), np.zeros(n, np.uint8)
+    rc = lib().aro_poly_div_lnh(_p(a), _p(b), n, _p(q), _p(r))
+    if rc == 1:
+        return q, r
+    if rc == 0:
+        return None
+    raise ValueError("reference would panic")
+
+
+def deriv_lnh(c, n=256):
+    """poly::deriv_lnh (rs/src/poly_arith.rs:91-116) over n coefficients."""
+    c, out = _poly(c, n), np.zeros(n, np.uint8)
+    lib().aro_deriv_lnh(_p(c), n, _p(out))
+    return out
+
+
+def syndrome(n, T, received):
+    """compute_syndrome_scalar (rs/src/codec.rs:266-301): n entries, first T set."""
+    r = np.ascontiguousarray(np.asarray(received, dtype=np.uint8))
+    out = np.zeros(n, np.uint8)
+    lib().aro_syndrome_scalar(_p(r), n, T, _p(out))
+    return out
+
+
+def key_equation(syn, n, T, eea=False):
+    """solve_key_equation_hgcd (rs/src/codec.rs:212-239) or, with eea=True,
+    solve_key_equation_eea (rs/src/codec.rs:28-80): (v1, lambda) or None;
+    PANIC where the reference would panic."""
+    s = _poly(syn, n)
+    v1, lam = np.zeros(n, np.uint8), np.zeros(n, np.uint8)
+    rc = lib().aro_key_equation(_p(s), n, T, 1 if eea else 0, _p(v1), _p(lam))
+    if rc == 1:
+        return v1, lam
+    if rc == 0:
+        return None
+    return PANIC
+
+
+def formal_deriv(c, n=256):
+    """The formal derivative in the basis X (extension; not the reference's
+    deriv_lnh): D'[i] = xor over bits b not in i of D[i | 2^b]."""
+    c, out = _poly(c, n), np.zeros(n, np.uint8)
+    lib().aro_formal_deriv(_p(c), n, _p(out))
+    return out
+
+
+def decode_scalar(n, T, received, formal=False):
+    """Codec::decode_systematic_scalar (rs/src/codec.rs:341-420) on a copy.
+    Returns (rc, kind, word): rc 1/0 as the reference's bool (PANIC where it
+    would panic); kind 0 zero syndrome, 1 true after the key equation,
+    2 false, 3 panic.  formal=True evaluates errors with the formal
+    derivative instead of the reference's deriv_lnh (extension)."""
+    w = np.array(received, dtype=np.uint8)
+    kind = ctypes.c_int(0)
+    rc = lib().aro_decode_systematic_scalar(n, T, _p(w), ctypes.byref(kind), 1 if formal else 0)
+    return rc, kind.value, w
+
+
+def decode_sharded(n, T, received, L, formal=False):
+    """Each byte column of the n x L codeword decoded as decode_systematic_scalar
+    decodes one received word; in place.  Returns (rc, status per column)."""
+    status = np.zeros(L, np.uint8)
+    rc = lib().aro_decode_sharded(n, T, _p(received), L, _p(status), 1 if formal else 0)
+    return rc, status
