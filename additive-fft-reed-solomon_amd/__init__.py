@@ -36,6 +36,7 @@ ARS_E_UNSUPPORTED = -7
 ARS_E_BUDGET = -8
 ARS_E_STATE = -9
 ARS_ENC_ACCUMULATE = 1
+ARS_DEC_FORMAL_DERIVATIVE = 1
 
 _ERRNAMES = {v: k for k, v in dict(globals()).items() if k.startswith("ARS_E_")}
 
@@ -52,6 +53,7 @@ SIGNATURES = {
     "ars_encode": (_i, [_vp, _u32, _u32, _vp, _vp, _sz, _u32, _vp]),
     "ars_recover": (_i, [_vp, _u32, _u32, _vp, _sz, _vp, _u32, _vp]),
     "ars_recompute_data": (_i, [_vp, _u32, _u32, _vp, _sz, _vp]),
+    "ars_decode": (_i, [_vp, _u32, _u32, _vp, _sz, _u32, _vp, ctypes.POINTER(ctypes.c_uint64), _vp]),
     "ars_recover_plan_size": (_sz, []),
     "ars_recover_plan": (_i, [_vp, _u32, _u32, _vp, _u32, _vp, _vp]),
     "ars_recover_with_plan": (_i, [_vp, _u32, _u32, _vp, _sz, _vp, _vp]),
@@ -211,6 +213,13 @@ class Context:
     def recompute_data(self, n, k, d_received, shard_len, stream=None):
         return _check(lib().ars_recompute_data(self.handle, n, k, d_received, shard_len,
                                                _stream_ptr(stream)))
+
+    def decode(self, n, k, d_received, shard_len, flags=0, d_status=None, stream=None):
+        """ars_decode: returns (rc, (clean, decoded, failed) column counts)."""
+        counts = (ctypes.c_uint64 * 3)()
+        rc = _check(lib().ars_decode(self.handle, n, k, d_received, shard_len, flags, d_status, counts,
+                                     _stream_ptr(stream)))
+        return rc, tuple(int(c) for c in counts)
 
 
 class Batch:
@@ -435,6 +444,31 @@ class Codec:
             rc = _check(lib().ars_recover_host(self.ctx.handle, self.n, self.k, rp, shard_len,
                                                pos.ctypes.data if len(pos) else None, len(pos)))
         return rc == ARS_TRUE
+
+    def decode_systematic_sharded(self, received, shard_len=None, formal_derivative=False,
+                                  status=None, stream=None):
+        """Every byte column decoded as Codec::decode_systematic_scalar
+        (rs/src/codec.rs:341-420) decodes one received word, in place, on a
+        device buffer.  Returns the reference's bool for the whole codeword
+        (True iff no column returned false) and the column counts
+        (clean, decoded, failed).  formal_derivative=True evaluates errors
+        with the formal derivative instead of the reference's deriv_lnh.
+        `status`: optional device uint8 buffer of shard_len per-column codes."""
+        if shard_len is None:
+            shard_len = len(received) // self.n
+        if len(received) != self.n * shard_len:
+            raise ValueError("received size does not match (n, shard_len)")
+        rp, dev = _ptr(received)
+        if not dev:
+            raise ValueError("decode_systematic_sharded takes a device buffer")
+        sp = None
+        if status is not None:
+            sp, sdev = _ptr(status)
+            if not sdev or len(status) < shard_len:
+                raise ValueError("status must be a device buffer of shard_len bytes")
+        rc, counts = self.ctx.decode(self.n, self.k, rp, shard_len,
+                                     ARS_DEC_FORMAL_DERIVATIVE if formal_derivative else 0, sp, stream)
+        return rc == ARS_TRUE, counts
 
     def recompute_data_from_parity_sharded(self, received, workspace=None, shard_len=None,
                                            stream=None):

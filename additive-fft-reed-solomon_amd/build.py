@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["ars_kernels.hip", "ars_batch.hip", "ars_api.cpp", "ars_batch_host.cpp", "ars_serve.cpp", "ars_tune.cpp", "ars_host.cpp"]
+SOURCES = ["ars_kernels.hip", "ars_batch.hip", "ars_api.cpp", "ars_batch_host.cpp", "ars_serve.cpp", "ars_tune.cpp", "ars_host.cpp", "ars_decode.hip", "ars_decode_api.cpp"]
 
 
 def _compile(src):

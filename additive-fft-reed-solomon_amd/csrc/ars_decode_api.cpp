@@ -1,0 +1,117 @@
+// ars_decode_api.cpp -- ars_decode: sharded error decoding (see ars.h).
+//
+// 1. k_syndrome lists the columns whose syndrome is not zero (device).
+// 2. The host solves the key equation of each listed column on a worker pool
+//    (ars_keyeq.hpp) -- the step the reference runs as HGCD.
+// 3. k_decode_tail runs the root search and error evaluation of every listed
+//    column and applies the corrections in place (device).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/ars.h"
+#include "ars_ctx.hpp"
+#include "ars_decode.hpp"
+#include "ars_keyeq.hpp"
+#include "ars_work_pool.hpp"
+
+namespace {
+
+ars::WorkPool &solver_pool() {
+    static ars::WorkPool pool([] {
+        const unsigned hc = std::thread::hardware_concurrency();
+        return std::min(15u, hc > 1 ? hc - 1 : 0u);
+    }());
+    return pool;
+}
+std::mutex g_solver_mu;  // one job at a time on the pool
+
+bool aligned4(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; }
+
+struct Scratch {
+    void *p = nullptr;
+    hipStream_t s = nullptr;
+    ~Scratch() {
+        if (p) hipFreeAsync(p, s);
+    }
+};
+
+}  // namespace
+
+extern "C" int ars_decode(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_t L, uint32_t flags,
+                          uint8_t *d_status, uint64_t *counts, void *stream) {
+    if (!ctx || !d_received) return ars::set_error(ARS_E_NULL, "null context or buffer");
+    int rc = ars::check_code(n, k, L);
+    if (rc) return rc;
+    if (flags & ~ARS_DEC_FORMAL_DERIVATIVE) return ars::set_error(ARS_E_INVAL, "unknown decode flags");
+    const uint32_t T = n - k;
+    if (!ars::is_pow2(T) || T < 2)
+        return ars::set_error(ARS_E_UNSUPPORTED, "error decoding needs T = n - k a power of two >= 2");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return ars::hip_error(e, "hipSetDevice");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int tl = ars::ilog2(T);
+
+    // scratch: [count, failed] | cols[L] | syn[L * T]
+    const size_t off_cols = 256, off_syn = off_cols + ((4 * L + 255) & ~size_t(255));
+    Scratch scr;
+    scr.s = s;
+    if ((e = hipMallocAsync(&scr.p, off_syn + L * T, s)) != hipSuccess) return ars::hip_error(e, "decode scratch");
+    uint8_t *base = static_cast<uint8_t *>(scr.p);
+    uint32_t *d_count = reinterpret_cast<uint32_t *>(base), *d_failed = d_count + 1;
+    uint32_t *d_cols = reinterpret_cast<uint32_t *>(base + off_cols);
+    uint8_t *d_syn = base + off_syn;
+    if ((e = hipMemsetAsync(base, 0, 8, s)) != hipSuccess) return ars::hip_error(e, "decode reset");
+    if (d_status && (e = hipMemsetAsync(d_status, 0, L, s)) != hipSuccess) return ars::hip_error(e, "status reset");
+
+    const bool al = (L % 4 == 0) && aligned4(d_received);
+    ars::SynOut so{d_count, d_cols, d_syn};
+    if ((e = ars::launch_syndrome(tl, al, d_received, (uint32_t)L, n / T, ctx->d_tw, so, s)) != hipSuccess)
+        return ars::hip_error(e, "syndrome launch");
+    uint32_t listed = 0, failed = 0;
+    if ((e = hipMemcpyAsync(&listed, d_count, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess)
+        return ars::hip_error(e, "syndrome count");
+
+    if (listed) {
+        std::vector<uint8_t> syn((size_t)listed * T);
+        if ((e = hipMemcpyAsync(syn.data(), d_syn, syn.size(), hipMemcpyDeviceToHost, s)) != hipSuccess ||
+            (e = hipStreamSynchronize(s)) != hipSuccess)
+            return ars::hip_error(e, "syndrome copy");
+        std::vector<ars::DecPlan> plans(listed);
+        const bool formal = flags & ARS_DEC_FORMAL_DERIVATIVE;
+        {
+            std::lock_guard<std::mutex> lk(g_solver_mu);
+            const size_t per = 16, pieces = (listed + per - 1) / per;
+            solver_pool().run(pieces, [&](size_t pi) {
+                const size_t end = std::min<size_t>(listed, (pi + 1) * per);
+                for (size_t i = pi * per; i < end; ++i)
+                    if (!ars::solve_key_equation(&syn[i * T], n, T, formal, plans[i])) plans[i].deg = -1;
+            });
+        }
+        Scratch pscr;
+        pscr.s = s;
+        if ((e = hipMallocAsync(&pscr.p, plans.size() * sizeof(ars::DecPlan), s)) != hipSuccess)
+            return ars::hip_error(e, "decode plans");
+        if ((e = hipMemcpyAsync(pscr.p, plans.data(), plans.size() * sizeof(ars::DecPlan), hipMemcpyHostToDevice,
+                                s)) != hipSuccess)
+            return ars::hip_error(e, "decode plans copy");
+        if ((e = ars::launch_decode_tail(d_received, (uint32_t)L, n, tl, d_cols,
+                                         static_cast<const ars::DecPlan *>(pscr.p), listed, d_status, d_failed,
+                                         ctx->d_gtab, ctx->d_tw, s)) != hipSuccess)
+            return ars::hip_error(e, "decode launch");
+        if ((e = hipMemcpyAsync(&failed, d_failed, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+            (e = hipStreamSynchronize(s)) != hipSuccess)
+            return ars::hip_error(e, "decode result");
+    }
+    if (counts) {
+        counts[0] = L - listed;
+        counts[1] = listed - failed;
+        counts[2] = failed;
+    }
+    return failed ? ARS_FALSE : ARS_TRUE;
+}

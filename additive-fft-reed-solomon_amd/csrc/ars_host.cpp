@@ -17,13 +17,13 @@
 #include <vector>
 
 #include "../../include/ars.h"
-#include "ars_copy_pool.hpp"
+#include "ars_work_pool.hpp"
 #include "ars_ctx.hpp"
 #include "ars_kernels.hpp"
 
 namespace {
 
-using ars::CopyPool;
+using ars::WorkPool;
 
 constexpr int kSlots = 3;
 constexpr size_t kChunkBytes = 16u << 20;  // device bytes per chunk (all n rows)
@@ -43,7 +43,7 @@ struct Pipe {
     std::mutex mu;
     Slot slots[kSlots];
     ars::RecPrep *plan = nullptr;
-    CopyPool *pool = nullptr;
+    WorkPool *pool = nullptr;
     bool ready = false;
 };
 
@@ -79,7 +79,7 @@ int init_pipe(ars_ctx *ctx, Pipe *p) {
     }
     if ((e = hipMalloc(&p->plan, sizeof(ars::RecPrep))) != hipSuccess) return ars::hip_error(e, "host pipeline plan");
     (void)ctx;
-    if (!p->pool) p->pool = new (std::nothrow) CopyPool(copy_workers());
+    if (!p->pool) p->pool = new (std::nothrow) WorkPool(copy_workers());
     if (!p->pool) return ars::set_error(ARS_E_NOMEM, "host pipeline workers");
     p->ready = true;
     return ARS_OK;
@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(256) k_pack_rows(const uint8_t *__restrict__ s
 
 // Row copies between a pitch-L caller buffer (columns [c0, c0 + w)) and a
 // pitch-w staging buffer, split into pieces of about 1 MiB over the pool.
-void copy_rows(CopyPool *pool, uint8_t *pitched, uint8_t *staged, size_t L, size_t c0, size_t w, uint32_t count,
+void copy_rows(WorkPool *pool, uint8_t *pitched, uint8_t *staged, size_t L, size_t c0, size_t w, uint32_t count,
                bool to_staging) {
     const size_t per = std::max<size_t>(1, (size_t(1) << 20) / std::max<size_t>(w, 1));
     const size_t pieces = (count + per - 1) / per;
@@ -133,7 +133,7 @@ void copy_rows(CopyPool *pool, uint8_t *pitched, uint8_t *staged, size_t L, size
     });
 }
 // Packed staging row i -> caller row rows[i].
-void copy_packed(CopyPool *pool, uint8_t *dst, const uint8_t *packed, size_t L, size_t c0, size_t w,
+void copy_packed(WorkPool *pool, uint8_t *dst, const uint8_t *packed, size_t L, size_t c0, size_t w,
                  const uint8_t *rows, uint32_t count) {
     const size_t per = std::max<size_t>(1, (size_t(1) << 20) / std::max<size_t>(w, 1));
     const size_t pieces = (count + per - 1) / per;
@@ -142,10 +142,10 @@ void copy_packed(CopyPool *pool, uint8_t *dst, const uint8_t *packed, size_t L, 
         for (size_t i = pi * per; i < end; ++i) std::memcpy(dst + (size_t)rows[i] * L + c0, packed + i * w, w);
     });
 }
-void gather(CopyPool *pool, uint8_t *dst, const uint8_t *src, size_t L, size_t c0, size_t w, uint32_t rows) {
+void gather(WorkPool *pool, uint8_t *dst, const uint8_t *src, size_t L, size_t c0, size_t w, uint32_t rows) {
     copy_rows(pool, const_cast<uint8_t *>(src), dst, L, c0, w, rows, true);
 }
-void scatter(CopyPool *pool, uint8_t *dst, const uint8_t *src, size_t L, size_t c0, size_t w, uint32_t rows) {
+void scatter(WorkPool *pool, uint8_t *dst, const uint8_t *src, size_t L, size_t c0, size_t w, uint32_t rows) {
     copy_rows(pool, dst, const_cast<uint8_t *>(src), L, c0, w, rows, false);
 }
 

@@ -1,5 +1,6 @@
-// ars_copy_pool.hpp -- a small persistent worker pool for the host-slice
-// pipeline's row copies between pageable caller memory and pinned staging.
+// ars_work_pool.hpp -- a small persistent worker pool for host-side work:
+// the host-slice pipeline's row copies between pageable caller memory and
+// pinned staging, and the per-column key equations of error decoding.
 // HIP-free so that it builds (and is tested under ThreadSanitizer) with g++.
 #pragma once
 
@@ -16,12 +17,12 @@ namespace ars {
 
 // Persistent workers that run fn(i) for i in [0, count) together with the
 // calling thread.  One job at a time (the pipe's mutex serialises callers).
-class CopyPool {
+class WorkPool {
   public:
-    explicit CopyPool(unsigned workers) {
+    explicit WorkPool(unsigned workers) {
         for (unsigned i = 0; i < workers; ++i) threads_.emplace_back([this] { loop(); });
     }
-    ~CopyPool() {
+    ~WorkPool() {
         {
             std::lock_guard<std::mutex> lk(mu_);
             stop_ = true;

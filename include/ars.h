@@ -107,6 +107,31 @@ int ars_recompute_data(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received
                        size_t shard_len, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Sharded error decoding.  The reference decodes errors (not only erasures)
+ * one received word at a time, Codec::decode_systematic_scalar
+ * (rs/src/codec.rs:341-420), and has no sharded form (README.md:58).  Here
+ * every byte column of the n x L codeword is one received word, decoded in
+ * place exactly as decode_systematic_scalar decodes it: left as it is when
+ * its syndrome is zero or when the reference returns false, corrected
+ * otherwise.  The syndromes, the root search and the error values run on the
+ * GPU; the key equation of each column with a non-zero syndrome is solved on
+ * the host, so the call is synchronous on `stream`.
+ *
+ * T = n - k must be a power of two >= 2 (the reference's decoder; its HGCD
+ * needs t_log >= 1).  flags: 0 reproduces the reference byte for byte,
+ * including its error values, which use poly::deriv_lnh
+ * (rs/src/poly_arith.rs:91-116) where the formal derivative belongs and are
+ * wrong for many patterns of up to T/2 errors; ARS_DEC_FORMAL_DERIVATIVE uses
+ * the formal derivative and corrects every pattern of up to T/2 errors.
+ * d_status (optional, device, L bytes): per column 0 syndrome zero, 1 the
+ * reference's true after the key equation, 2 the reference's false.
+ * counts (optional, host, 3 values): columns with status 0, 1, 2.
+ * Returns ARS_TRUE when no column returned false, ARS_FALSE otherwise. */
+#define ARS_DEC_FORMAL_DERIVATIVE 1u
+int ars_decode(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_t shard_len,
+               uint32_t flags, uint8_t *d_status, uint64_t *counts, void *stream);
+
+/* ------------------------------------------------------------------------
  * Batches: many codewords of uneven (n, k), shard lengths and erasure
  * patterns in one persistent kernel launch, replayed from a hipGraph.
  * No counterpart in the reference (it processes one codeword per call); each
@@ -240,8 +265,9 @@ int ars_tune_set(ars_ctx *ctx, uint32_t op, uint32_t log2_size, uint32_t quads, 
 int ars_batch_autotune(ars_batch *b, ars_job *jobs, uint32_t count, uint32_t *best_blocks_per_cu,
                        void *stream);
 
-/* Host-slice variants: same semantics on host buffers, synchronous; they
- * stage through device memory (PCIe-bound, not the HBM roofline path). */
+/* Host-slice variants: same semantics on host buffers, synchronous.  They
+ * stream the slices through the GPU in column chunks over pinned staging
+ * (PCIe-bound, not the HBM roofline path). */
 int ars_encode_host(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *message,
                     uint8_t *parity, size_t shard_len, uint32_t flags);
 int ars_recover_host(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *received, size_t shard_len,

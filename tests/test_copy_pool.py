@@ -1,5 +1,5 @@
-"""CPU unit test of the host-slice pipeline's copy worker pool
-(csrc/ars_copy_pool.hpp), built with g++ under ThreadSanitizer: every index of
+"""CPU unit test of the host worker pool
+(csrc/ars_work_pool.hpp), built with g++ under ThreadSanitizer: every index of
 every job runs exactly once, jobs run back to back with no overlap, and pools
 of 0..8 workers start and stop cleanly."""
 import os
@@ -13,7 +13,7 @@ import _pkg
 CSRC = os.path.join(_pkg.PKG_DIR, "csrc")
 
 DRIVER = r'''
-#include "ars_copy_pool.hpp"
+#include "ars_work_pool.hpp"
 #include <cstdio>
 #include <vector>
 using namespace ars;
@@ -21,7 +21,7 @@ using namespace ars;
 int main() {
     int bad = 0;
     for (unsigned workers = 0; workers <= 8; ++workers) {
-        CopyPool pool(workers);
+        WorkPool pool(workers);
         for (size_t job = 0; job < 300; ++job) {
             const size_t count = (job * 37) % 97;
             std::vector<std::atomic<int>> hits(count);
