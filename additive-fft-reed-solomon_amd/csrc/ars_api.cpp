@@ -106,6 +106,7 @@ int ars_destroy(ars_ctx *ctx) {
     if (!ctx) return ARS_OK;
     hipSetDevice(ctx->device);
     ars::release_host_pipe(ctx);
+    ars::release_decode_state(ctx);
     hipFree(ctx->d_tw);
     hipFree(ctx->d_gtab);
     delete ctx;

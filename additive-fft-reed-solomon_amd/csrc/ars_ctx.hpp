@@ -24,5 +24,6 @@ int hip_error(hipError_t e, const char *where);
 int check_code(uint32_t n, uint32_t k, size_t L);
 bool is_pow2(uint32_t v);
 int ilog2(uint32_t v);
-void release_host_pipe(ars_ctx *ctx);  // ars_host.cpp
+void release_host_pipe(ars_ctx *ctx);     // ars_host.cpp
+void release_decode_state(ars_ctx *ctx);  // ars_decode_api.cpp
 }  // namespace ars

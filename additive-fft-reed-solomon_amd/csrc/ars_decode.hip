@@ -93,13 +93,13 @@ __global__ void __launch_bounds__(256) k_syndrome_small(const uint8_t *__restric
 // h*16 + q*GB + m).  The lanes of a quad OR their rows into LDS, lane q = 0
 // lists the quad's non-zero columns, and every lane then writes its rows of
 // those columns.
-template <int TL, int PQ, bool AL>
+template <int TL, int PQ, int NBUF, bool AL>
 __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
     k_syndrome_tile(const uint8_t *__restrict__ rx, uint32_t L, uint32_t chunks,
                     const MulTab *__restrict__ g_tw, SynOut so) {
     using TT = Tile<TL, PQ>;
     constexpr int T = TT::N, Q = TT::Q, GB = TT::GB, RB = TT::RB;
-    __shared__ uint32_t tile[2 * TT::WORDS];
+    __shared__ uint32_t tile[NBUF * TT::WORDS];
     __shared__ MulTab tw[kTwiddleEntries];
     __shared__ uint32_t s_or[PQ], s_base[PQ];
     stage_tw(tw, g_tw);
@@ -118,16 +118,24 @@ __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
 #pragma unroll
         for (int j = 0; j < 16; ++j) x[j] = io.ld_in(B + q * 16, j);
         run_levels<4, 0, 0, 4, true, false>(x, B + q * 16, tw);
-        uint32_t *tb = tile + (c & 1) * TT::WORDS;
+        uint32_t *tb = tile + (NBUF == 2 ? (c & 1) * TT::WORDS : 0);
+        if (NBUF == 1 && c > 0) __syncthreads();
         TT::put_a(tb, x, q, p);
         __syncthreads();
         TT::get_b(tb, y, q, p);
+        // chunk 0 sits at beta = 0: its phase-B blocks at offset 0 have the
+        // zero twiddle, XOR-only at compile time
+        if (c == 0) {
 #pragma unroll
-        for (int m = 0; m < GB; ++m) {
-            run_levels<RB, 4, 4, TL, true, false>(y[m], B, tw);
+            for (int m = 0; m < GB; ++m) run_levels<RB, 4, 4, TL, true, true>(y[m], 0u, tw);
+        } else {
+#pragma unroll
+            for (int m = 0; m < GB; ++m) run_levels<RB, 4, 4, TL, true, false>(y[m], B, tw);
+        }
+#pragma unroll
+        for (int m = 0; m < GB; ++m)
 #pragma unroll
             for (int h = 0; h < Q; ++h) acc[m][h] ^= y[m][h];
-        }
     }
     uint32_t orw = 0;
 #pragma unroll
@@ -241,9 +249,11 @@ hipError_t syn_one(const uint8_t *rx, uint32_t L, uint32_t chunks, const MulTab 
         hipLaunchKernelGGL((k_syndrome_small<TL, AL>), dim3(ceil_div(quads, 256)), dim3(256), 0, s, rx, L, chunks,
                            tw, so);
     } else {
-        constexpr int PQ = 64;
-        hipLaunchKernelGGL((k_syndrome_tile<TL, PQ, AL>), dim3(ceil_div(quads, PQ)), dim3(Tile<TL, PQ>::THREADS),
-                           0, s, rx, L, chunks, tw, so);
+        // the encode tile's measured best shapes: 32 quads and one LDS buffer
+        // at 128 points, 64 quads and two buffers below
+        constexpr int PQ = TL >= 7 ? 32 : 64, NB = TL >= 7 ? 1 : 2;
+        hipLaunchKernelGGL((k_syndrome_tile<TL, PQ, NB, AL>), dim3(ceil_div(quads, PQ)),
+                           dim3(Tile<TL, PQ>::THREADS), 0, s, rx, L, chunks, tw, so);
     }
     return hipGetLastError();
 }

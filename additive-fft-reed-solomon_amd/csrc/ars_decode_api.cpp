@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -32,15 +33,50 @@ std::mutex g_solver_mu;  // one job at a time on the pool
 
 bool aligned4(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; }
 
-struct Scratch {
-    void *p = nullptr;
-    hipStream_t s = nullptr;
-    ~Scratch() {
-        if (p) hipFreeAsync(p, s);
-    }
+// Per-context decode scratch, kept between calls and grown on demand:
+// device [listed, failed] counters | column list | syndromes, the device
+// plans, and a pinned copy of the counters.  Calls on one context are
+// serialised by `mu`.
+struct DecState {
+    std::mutex mu;
+    uint8_t *dev = nullptr;
+    size_t dev_bytes = 0;
+    ars::DecPlan *plans = nullptr;
+    size_t plan_count = 0;
+    uint32_t *h_counts = nullptr;
 };
 
+std::mutex g_states_mu;
+std::vector<std::pair<ars_ctx *, DecState *>> g_states;
+
+DecState *state_for(ars_ctx *ctx) {
+    std::lock_guard<std::mutex> lk(g_states_mu);
+    for (auto &p : g_states)
+        if (p.first == ctx) return p.second;
+    DecState *st = new (std::nothrow) DecState;
+    if (st) g_states.emplace_back(ctx, st);
+    return st;
+}
+
 }  // namespace
+
+void ars::release_decode_state(ars_ctx *ctx) {
+    DecState *st = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_states_mu);
+        for (auto it = g_states.begin(); it != g_states.end(); ++it)
+            if (it->first == ctx) {
+                st = it->second;
+                g_states.erase(it);
+                break;
+            }
+    }
+    if (!st) return;
+    hipFree(st->dev);
+    hipFree(st->plans);
+    hipHostFree(st->h_counts);
+    delete st;
+}
 
 extern "C" int ars_decode(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_t L, uint32_t flags,
                           uint8_t *d_status, uint64_t *counts, void *stream) {
@@ -56,12 +92,24 @@ extern "C" int ars_decode(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_recei
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int tl = ars::ilog2(T);
 
-    // scratch: [count, failed] | cols[L] | syn[L * T]
+    DecState *st = state_for(ctx);
+    if (!st) return ars::set_error(ARS_E_NOMEM, "decode state allocation failed");
+    std::lock_guard<std::mutex> lk(st->mu);
+    // scratch: [listed, failed] | cols[L] | syn[L * T]
     const size_t off_cols = 256, off_syn = off_cols + ((4 * L + 255) & ~size_t(255));
-    Scratch scr;
-    scr.s = s;
-    if ((e = hipMallocAsync(&scr.p, off_syn + L * T, s)) != hipSuccess) return ars::hip_error(e, "decode scratch");
-    uint8_t *base = static_cast<uint8_t *>(scr.p);
+    const size_t need = off_syn + L * T;
+    if (st->dev_bytes < need) {
+        hipFree(st->dev);
+        st->dev = nullptr;
+        st->dev_bytes = 0;
+        if ((e = hipMalloc(reinterpret_cast<void **>(&st->dev), need)) != hipSuccess)
+            return ars::hip_error(e, "decode scratch");
+        st->dev_bytes = need;
+    }
+    if (!st->h_counts &&
+        (e = hipHostMalloc(reinterpret_cast<void **>(&st->h_counts), 8, hipHostMallocDefault)) != hipSuccess)
+        return ars::hip_error(e, "decode counters");
+    uint8_t *base = st->dev;
     uint32_t *d_count = reinterpret_cast<uint32_t *>(base), *d_failed = d_count + 1;
     uint32_t *d_cols = reinterpret_cast<uint32_t *>(base + off_cols);
     uint8_t *d_syn = base + off_syn;
@@ -73,9 +121,10 @@ extern "C" int ars_decode(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_recei
     if ((e = ars::launch_syndrome(tl, al, d_received, (uint32_t)L, n / T, ctx->d_tw, so, s)) != hipSuccess)
         return ars::hip_error(e, "syndrome launch");
     uint32_t listed = 0, failed = 0;
-    if ((e = hipMemcpyAsync(&listed, d_count, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+    if ((e = hipMemcpyAsync(st->h_counts, d_count, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
         (e = hipStreamSynchronize(s)) != hipSuccess)
         return ars::hip_error(e, "syndrome count");
+    listed = st->h_counts[0];
 
     if (listed) {
         std::vector<uint8_t> syn((size_t)listed * T);
@@ -93,20 +142,24 @@ extern "C" int ars_decode(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_recei
                     if (!ars::solve_key_equation(&syn[i * T], n, T, formal, plans[i])) plans[i].deg = -1;
             });
         }
-        Scratch pscr;
-        pscr.s = s;
-        if ((e = hipMallocAsync(&pscr.p, plans.size() * sizeof(ars::DecPlan), s)) != hipSuccess)
-            return ars::hip_error(e, "decode plans");
-        if ((e = hipMemcpyAsync(pscr.p, plans.data(), plans.size() * sizeof(ars::DecPlan), hipMemcpyHostToDevice,
-                                s)) != hipSuccess)
+        if (st->plan_count < listed) {
+            hipFree(st->plans);
+            st->plans = nullptr;
+            st->plan_count = 0;
+            if ((e = hipMalloc(reinterpret_cast<void **>(&st->plans), listed * sizeof(ars::DecPlan))) != hipSuccess)
+                return ars::hip_error(e, "decode plans");
+            st->plan_count = listed;
+        }
+        if ((e = hipMemcpyAsync(st->plans, plans.data(), plans.size() * sizeof(ars::DecPlan),
+                                hipMemcpyHostToDevice, s)) != hipSuccess)
             return ars::hip_error(e, "decode plans copy");
-        if ((e = ars::launch_decode_tail(d_received, (uint32_t)L, n, tl, d_cols,
-                                         static_cast<const ars::DecPlan *>(pscr.p), listed, d_status, d_failed,
-                                         ctx->d_gtab, ctx->d_tw, s)) != hipSuccess)
+        if ((e = ars::launch_decode_tail(d_received, (uint32_t)L, n, tl, d_cols, st->plans, listed, d_status,
+                                         d_failed, ctx->d_gtab, ctx->d_tw, s)) != hipSuccess)
             return ars::hip_error(e, "decode launch");
-        if ((e = hipMemcpyAsync(&failed, d_failed, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+        if ((e = hipMemcpyAsync(st->h_counts + 1, d_failed, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
             (e = hipStreamSynchronize(s)) != hipSuccess)
             return ars::hip_error(e, "decode result");
+        failed = st->h_counts[1];
     }
     if (counts) {
         counts[0] = L - listed;

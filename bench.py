@@ -252,6 +252,72 @@ def bench_host(ars, torch, np, n, k, L, e, reps=5, seed=46):
             "pinned_h2d_GBps": round(h2d, 1), "pinned_d2h_GBps": round(d2h, 1)}
 
 
+def bench_decode(ars, torch, np, n, k, L, reps=5, seed=47):
+    """Sharded error decoding of the same (n, k) codeword (formal derivative):
+    a clean scrub at shard_len L (syndrome pass only), 1 % of the columns with
+    <= T/2 errors each at L, and whole-shard corruption (4 shards, so every
+    column needs the host key equation) at L/16.  Wall-clock per call
+    (the call is synchronous); bytes counted as n*L read."""
+    T = n - k
+    c = ars.Codec(n, k)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    msg = torch.randint(0, 256, (k * L,), dtype=torch.uint8, device="cuda", generator=g)
+    par = torch.zeros(T * L, dtype=torch.uint8, device="cuda")
+    c.encode_systematic_sharded(msg, par, None, L, accumulate=False)
+    cw = torch.cat([par, msg]).view(n, L)
+    rng = np.random.default_rng(seed)
+
+    def timed(rx_fn, length):
+        best = None
+        for _ in range(reps):
+            rx = rx_fn()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            ok, counts = c.decode_systematic_sharded(rx.view(-1), length, formal_derivative=True)
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        return best, ok, counts
+
+    t_clean, ok, counts = timed(lambda: cw.clone(), L)
+    assert ok and counts == (L, 0, 0)
+    cols = torch.from_numpy(np.sort(rng.permutation(L)[:L // 100])).cuda()
+    sub = cw[:, cols].cpu().numpy()
+    for j in range(sub.shape[1]):
+        ne = int(rng.integers(1, T // 2 + 1))
+        sub[rng.permutation(n)[:ne], j] ^= rng.integers(1, 256, ne, dtype=np.uint8)
+    bad = torch.from_numpy(sub).cuda()
+
+    def sparse():
+        rx = cw.clone()
+        rx[:, cols] = bad
+        return rx
+
+    t_sparse, ok, counts = timed(sparse, L)
+    assert ok and counts[1] == len(cols)
+    Ld = L // 16
+    cwd = cw[:, :Ld].contiguous()
+    shards = torch.from_numpy(rng.permutation(n)[:4]).cuda()
+    noise = torch.randint(1, 256, (4, Ld), dtype=torch.uint8, device="cuda", generator=g)
+
+    def dense():
+        rx = cwd.clone()
+        rx[shards] ^= noise
+        return rx
+
+    t_dense, ok, counts = timed(dense, Ld)
+    assert ok and counts == (0, Ld, 0)
+    return {"workload": f"(n={n},k={k}) formal-derivative decode",
+            "clean_scrub": {"shard_len": L, "ms": round(t_clean * 1e3, 3),
+                            "GBps": round(n * L / t_clean / 1e9, 1)},
+            "sparse_1pct_columns": {"shard_len": L, "bad_columns": int(len(cols)),
+                                    "ms": round(t_sparse * 1e3, 3),
+                                    "GBps": round(n * L / t_sparse / 1e9, 1)},
+            "dense_4_shards": {"shard_len": Ld, "bad_columns": Ld, "ms": round(t_dense * 1e3, 3),
+                               "columns_per_s": round(Ld / t_dense, 1),
+                               "GBps": round(n * Ld / t_dense / 1e9, 2)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,6 +328,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-host", action="store_true")
+    ap.add_argument("--no-decode", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every figure is "
@@ -398,6 +465,9 @@ def main():
     if not args.no_host:
         out["host_slices"] = bench_host(ars, torch, np, n, k, L, e)
         print(f"host slices: {out['host_slices']}", file=sys.stderr, flush=True)
+    if not args.no_decode:
+        out["decode"] = bench_decode(ars, torch, np, n, k, L)
+        print(f"decode: {out['decode']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, k, L, e, args.cpu_seconds)
     print(json.dumps(out), flush=True)
