@@ -12,6 +12,7 @@
  * scale loops change.  Zero twiddles take the XOR-only path that the
  * reference's generated unrolled kernels take (rs/build.rs:36-46, 86-96).
  */
+#define _POSIX_C_SOURCE 199309L /* clock_gettime for the timing helper */
 #include <stdint.h>
 #include <stddef.h>
 #include <string.h>
@@ -192,4 +193,62 @@ TGT int aro_gfni_recover_erasure_shards(size_t n, size_t T, uint8_t *received, u
     for (size_t i = 0; i < e; i++)
         scale_gfni(workspace + (size_t)pos[i] * L, received + (size_t)pos[i] * L, L, G_INV[denoms[i]]);
     return 1;
+}
+
+/* Timing helper for the CPU baseline (not part of the restatement): runs
+ * encode (zeroed parity) and recover (erased shards zeroed) of one seeded
+ * (n, T, L) codeword `reps` times on buffers allocated and touched once, as
+ * the reference's criterion benches do (rs/benches/), and writes the
+ * median nanoseconds of each to out_ns[0], out_ns[1].  Returns 0, or -1 on
+ * bad arguments or allocation failure. */
+#include <stdlib.h>
+#include <time.h>
+
+static double now_ns(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec * 1e9 + (double)ts.tv_nsec;
+}
+static int cmp_d(const void *a, const void *b) {
+    double x = *(const double *)a, y = *(const double *)b;
+    return (x > y) - (x < y);
+}
+
+int aro_gfni_time_codec(size_t n, size_t T, size_t L, const uint8_t *pos, size_t e, int reps, double *out_ns) {
+    if (!aro_gfni_supported() || reps < 1 || n < 2 || n > 256 || T >= n || e > T) return -1;
+    g_init();
+    size_t k = n - T;
+    uint8_t *msg = malloc(k * L), *par = malloc(T * L), *rx = malloc(n * L), *ws = malloc(n * L);
+    double *te = malloc(sizeof(double) * reps), *tr = malloc(sizeof(double) * reps);
+    int rc = -1;
+    if (!msg || !par || !rx || !ws || !te || !tr) goto done;
+    uint64_t s = 0x9e3779b97f4a7c15ull;
+    for (size_t i = 0; i < k * L; i++) {
+        s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+        msg[i] = (uint8_t)s;
+    }
+    memset(ws, 1, n * L);
+    memset(rx, 1, n * L);
+    for (int r = 0; r < reps; r++) {
+        memset(par, 0, T * L);
+        double a = now_ns();
+        if (aro_gfni_encode_systematic_sharded(n, T, msg, par, ws, L)) goto done;
+        double b = now_ns();
+        memcpy(rx, par, T * L);
+        memcpy(rx + T * L, msg, k * L);
+        for (size_t i = 0; i < e; i++) memset(rx + (size_t)pos[i] * L, 0, L);
+        double c = now_ns();
+        if (aro_gfni_recover_erasure_shards(n, T, rx, ws, L, pos, e) != 1) goto done;
+        double d = now_ns();
+        te[r] = b - a;
+        tr[r] = d - c;
+    }
+    qsort(te, reps, sizeof(double), cmp_d);
+    qsort(tr, reps, sizeof(double), cmp_d);
+    out_ns[0] = te[reps / 2];
+    out_ns[1] = tr[reps / 2];
+    rc = 0;
+done:
+    free(msg); free(par); free(rx); free(ws); free(te); free(tr);
+    return rc;
 }

@@ -61,6 +61,9 @@ def lib():
         L.aro_ext_recover_lagrange.argtypes = [_sz, _sz, _u8p, _sz, _u8p, _sz]
         L.aro_gfni_supported.restype = ctypes.c_int
         L.aro_gfni_supported.argtypes = []
+        L.aro_gfni_time_codec.restype = ctypes.c_int
+        L.aro_gfni_time_codec.argtypes = [_sz, _sz, _sz, _u8p, _sz, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_double)]
         L.aro_decode_systematic_scalar.restype = ctypes.c_int
         L.aro_decode_systematic_scalar.argtypes = [_sz, _sz, _u8p, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         L.aro_decode_sharded.restype = ctypes.c_int
@@ -124,14 +127,17 @@ def ifft_sharded(shards, L, k, beta, gfni=False):
     f(_p(shards), L, k, beta)
 
 
-def encode(n, T, message, L, parity=None, gfni=False):
+def encode(n, T, message, L, parity=None, gfni=False, workspace=None):
     """Codec::encode_systematic_sharded (rs/src/codec.rs:241-265).
 
     `parity` is XOR-accumulated into (the reference's behaviour); a fresh
-    zeroed buffer is used when it is None.  Returns the parity array."""
+    zeroed buffer is used when it is None.  `workspace` (T*L bytes, contents
+    irrelevant) may be passed to reuse one, as the reference's callers do.
+    Returns the parity array."""
     if parity is None:
         parity = np.zeros(T * L, np.uint8)
-    ws = np.zeros(T * L, np.uint8)
+    ws = np.zeros(T * L, np.uint8) if workspace is None else workspace
+    assert ws.size >= T * L
     f = lib().aro_gfni_encode_systematic_sharded if gfni else lib().aro_encode_systematic_sharded
     rc = f(n, T, _p(message), _p(parity), _p(ws), L)
     if rc != 0:
@@ -139,10 +145,12 @@ def encode(n, T, message, L, parity=None, gfni=False):
     return parity
 
 
-def recover(n, T, received, L, positions, gfni=False):
+def recover(n, T, received, L, positions, gfni=False, workspace=None):
     """Codec::recover_erasure_shards (rs/src/codec.rs:479-569); in place.
-    Returns the reference's bool as 1/0 (negative for rejected input)."""
-    ws = np.zeros(n * L, np.uint8)
+    Returns the reference's bool as 1/0 (negative for rejected input).
+    `workspace` (n*L bytes) may be passed to reuse one."""
+    ws = np.zeros(n * L, np.uint8) if workspace is None else workspace
+    assert ws.size >= n * L
     pos = np.ascontiguousarray(np.asarray(positions, dtype=np.uint8))
     f = lib().aro_gfni_recover_erasure_shards if gfni else lib().aro_recover_erasure_shards
     return f(n, T, _p(received), _p(ws), L, _p(pos) if len(pos) else None, len(pos))
@@ -273,3 +281,12 @@ def decode_sharded(n, T, received, L, formal=False):
     status = np.zeros(L, np.uint8)
     rc = lib().aro_decode_sharded(n, T, _p(received), L, _p(status), 1 if formal else 0)
     return rc, status
+
+
+def gfni_time_codec(n, T, L, positions, reps):
+    """Median ns of one GFNI encode and one recover, timed inside C on
+    buffers allocated once (CPU baseline only).  None without GFNI."""
+    pos = np.ascontiguousarray(np.asarray(positions, dtype=np.uint8))
+    out = (ctypes.c_double * 2)()
+    rc = lib().aro_gfni_time_codec(n, T, L, _p(pos), len(pos), reps, out)
+    return None if rc else (out[0], out[1])
