@@ -114,16 +114,47 @@ def cpu_baseline(n, k, L, e, seconds, threads=16):
     }
 
 
+def _profile_entry(fname, kernel):
+    """The committed rocprofv3 record (profiles/<fname>) of `kernel`, or of
+    the same kernel family (another tile variant) when that variant was not
+    profiled; (name, record) or (None, None)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", fname)) as f:
+            recs = json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
+        return None, None
+    if kernel in recs:
+        return kernel, recs[kernel]
+    fam = kernel.split(",")[0]
+    for name, rec in recs.items():
+        if name.startswith(fam):
+            return name, rec
+    return None, None
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE passes (profiles/traffic_256x128_1MiB.json), or None."""
-    path = os.path.join(ROOT, "profiles", "traffic_256x128_1MiB.json")
+    _, rec = _profile_entry("traffic_256x128_1MiB.json", kernel)
+    return rec["traffic_bytes"] if rec else None
+
+
+def valu_roofline(kernel, seconds):
+    """Secondary roofline: VALU lane-ops per launch of `kernel` (committed
+    SQ_INSTS_VALU x waves x 64, profiles/valu_256x128_1MiB.json) over the
+    launch time measured now, against 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz."""
     try:
-        with open(path) as f:
-            rec = json.load(f)["kernels"].get(kernel)
-        return rec["traffic_bytes"] if rec else None
+        with open(os.path.join(ROOT, "profiles", "valu_256x128_1MiB.json")) as f:
+            peak = json.load(f)["valu_peak_lane_ops_per_s"]
     except (OSError, ValueError, KeyError):
         return None
+    name, rec = _profile_entry("valu_256x128_1MiB.json", kernel)
+    if rec is None:
+        return None
+    achieved = rec["lane_ops_per_launch"] / seconds
+    return {"bound": "valu", "kernel": name, "achieved": round(achieved / 1e12, 2), "peak": round(peak / 1e12, 2),
+            "unit": "T lane-ops/s", "frac": round(achieved / peak, 4),
+            "lane_ops_per_launch": rec["lane_ops_per_launch"]}
 
 
 def bench_batched(ars, torch, np, stream, count=4096, seed=44, reps=3):
@@ -459,10 +490,13 @@ def main():
     del dst
     enc_gbps = n * L / s_enc / 1e9
     rec_gbps = (n + e) * L / s_rec / 1e9
+    # the tile variants the autotuner kept (quads per workgroup, LDS buffers)
+    enc_name = f"k_encode_tile<7, {tune_enc['quads']}, true, false, {tune_enc['buffers']}>"
+    rec_name = f"k_recover_tile<8, {tune_rec['quads']}, true, {tune_rec['buffers']}>"
     if s_rec >= s_enc:
-        dom, dom_bytes, dom_s = "k_recover_tile<8, 32, true, 1>", (n + e) * L, s_rec
+        dom, dom_bytes, dom_s = rec_name, (n + e) * L, s_rec
     else:
-        dom, dom_bytes, dom_s = "k_encode_tile<7, 64, true, false>", n * L, s_enc
+        dom, dom_bytes, dom_s = enc_name, n * L, s_enc
     achieved = dom_bytes / dom_s / 1e9
     traffic = measured_traffic(dom)
 
@@ -485,10 +519,13 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                      "traffic": traffic, "algorithmic_bytes": dom_bytes},
+        "valu_roofline": valu_roofline(dom, dom_s),
         "encode": {"GBps": round(enc_gbps, 1), "us": round(s_enc * 1e6, 2),
-                   "frac_hbm": round(enc_gbps / HBM_PEAK_GBPS, 4), "bytes": n * L},
+                   "frac_hbm": round(enc_gbps / HBM_PEAK_GBPS, 4), "bytes": n * L,
+                   "valu_roofline": valu_roofline(enc_name, s_enc)},
         "recover": {"GBps": round(rec_gbps, 1), "us": round(s_rec * 1e6, 2),
-                    "frac_hbm": round(rec_gbps / HBM_PEAK_GBPS, 4), "bytes": (n + e) * L},
+                    "frac_hbm": round(rec_gbps / HBM_PEAK_GBPS, 4), "bytes": (n + e) * L,
+                    "valu_roofline": valu_roofline(rec_name, s_rec)},
         "autotune": {"encode": tune_enc, "recover": tune_rec},
         "copy_reference": {"GBps": round(copy_gbps, 1), "bytes_read_plus_written": 2 * n * L,
                            "frac_hbm": round(copy_gbps / HBM_PEAK_GBPS, 4)},
