@@ -37,6 +37,7 @@ ARS_E_BUDGET = -8
 ARS_E_STATE = -9
 ARS_ENC_ACCUMULATE = 1
 ARS_DEC_FORMAL_DERIVATIVE = 1
+ARS_DEC_HOST_KEY_EQUATION = 2
 
 _ERRNAMES = {v: k for k, v in dict(globals()).items() if k.startswith("ARS_E_")}
 
@@ -446,14 +447,16 @@ class Codec:
         return rc == ARS_TRUE
 
     def decode_systematic_sharded(self, received, shard_len=None, formal_derivative=False,
-                                  status=None, stream=None):
+                                  status=None, stream=None, host_key_equation=False):
         """Every byte column decoded as Codec::decode_systematic_scalar
         (rs/src/codec.rs:341-420) decodes one received word, in place, on a
         device buffer.  Returns the reference's bool for the whole codeword
         (True iff no column returned false) and the column counts
         (clean, decoded, failed).  formal_derivative=True evaluates errors
         with the formal derivative instead of the reference's deriv_lnh.
-        `status`: optional device uint8 buffer of shard_len per-column codes."""
+        host_key_equation=True solves the key equations on the host (same
+        results).  `status`: optional device uint8 buffer of shard_len
+        per-column codes."""
         if shard_len is None:
             shard_len = len(received) // self.n
         if len(received) != self.n * shard_len:
@@ -466,8 +469,9 @@ class Codec:
             sp, sdev = _ptr(status)
             if not sdev or len(status) < shard_len:
                 raise ValueError("status must be a device buffer of shard_len bytes")
-        rc, counts = self.ctx.decode(self.n, self.k, rp, shard_len,
-                                     ARS_DEC_FORMAL_DERIVATIVE if formal_derivative else 0, sp, stream)
+        flags = (ARS_DEC_FORMAL_DERIVATIVE if formal_derivative else 0) | \
+            (ARS_DEC_HOST_KEY_EQUATION if host_key_equation else 0)
+        rc, counts = self.ctx.decode(self.n, self.k, rp, shard_len, flags, sp, stream)
         return rc == ARS_TRUE, counts
 
     def recompute_data_from_parity_sharded(self, received, workspace=None, shard_len=None,

@@ -9,11 +9,14 @@
 //                  column whose syndrome is not zero is appended to a list
 //                  with its T syndrome bytes.  Clean columns write nothing,
 //                  so a scrub of clean data only reads the codeword.
-//   k_decode_tail  one wave per listed column: the root search (one T-point
-//                  FFT of lambda per chunk, rs/src/codec.rs:383-396, the first
-//                  deg(lambda) roots in index order) and the error values
-//                  v(w_p) / lambda'(w_p) (rs/src/codec.rs:405-431), applied in
-//                  place when deg(lambda) roots were found.
+//   k_decode_columns  one wave per listed column: its key equation (the
+//                  same monomial-basis Euclid as the host solver, ars_keyeq.hpp),
+//                  then the root search (one T-point FFT of lambda per chunk,
+//                  rs/src/codec.rs:383-396, the first deg(lambda) roots in index
+//                  order) and the error values v(w_p) / lambda'(w_p)
+//                  (rs/src/codec.rs:405-431), applied in place when deg(lambda)
+//                  roots were found.
+//   k_decode_tail  the root-search half alone, for plans the host solved.
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
@@ -165,49 +168,51 @@ __device__ __forceinline__ uint8_t lmul(const uint8_t *ex, const uint8_t *lg, ui
     return (a && b) ? ex[lg[a] + lg[b]] : (uint8_t)0;
 }
 
-// One wave per listed column (blockIdx.x = list index).
-__global__ void __launch_bounds__(64) k_decode_tail(uint8_t *__restrict__ rx, uint32_t L, uint32_t n, uint32_t tl,
-                                                    const uint32_t *__restrict__ cols,
-                                                    const DecPlan *__restrict__ plans,
-                                                    uint8_t *__restrict__ status, uint32_t *__restrict__ failed,
-                                                    const uint8_t *__restrict__ gtab,
-                                                    const MulTab *__restrict__ g_tw) {
-    __shared__ uint8_t s_exp[512], s_log[256], s_tw[256];
-    __shared__ uint8_t ev[3][128];
-    __shared__ uint8_t rpos[256], rval[256];
+// LDS tables of a one-wave block: exp / log and the raw twiddles s_l(omega_o)
+// in the ars_gf.hpp layout (byte 1 of a MulTab's first table is c * 1 = c).
+struct WaveTabs {
+    uint8_t ex[512], lg[256], tw[256];
+};
+
+__device__ __forceinline__ void load_wave_tabs(WaveTabs &t, const uint8_t *__restrict__ gtab,
+                                               const MulTab *__restrict__ g_tw) {
+    for (uint32_t i = threadIdx.x; i < 512; i += 64) t.ex[i] = gtab[i];
+    for (uint32_t i = threadIdx.x; i < 256; i += 64) t.lg[i] = gtab[512 + i];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kTwiddleEntries; i += 64) t.tw[i] = (uint8_t)(g_tw[i].w[0] >> 8);
+}
+
+// Root search and error values of one column, by the wave of a 64-thread
+// block: lam, v, lp hold the first T coefficients (basis X) of lambda, v and
+// the derivative; dl = deg lambda (< 0: zero locator, nothing changes).  For
+// each chunk c, one forward T-point FFT of the three at beta = omega_{cT}
+// (rs/src/kernel/lut_kernel.rs:11-18, 28-54) in `ev`; the first dl roots in
+// index order are kept (rs/src/codec.rs:383-396) with their values
+// v(w_p) / lp(w_p) (0 where lp(w_p) = 0, rs/src/codec.rs:420-424).  If dl roots
+// were found the values are XORed into the column.  Returns true unless the
+// column fails (too few roots).
+__device__ bool tail_column(const WaveTabs &t, const uint8_t *lam, const uint8_t *v, const uint8_t *lp, int dl,
+                            uint32_t n, uint32_t tl, uint8_t *rx, uint32_t L, uint32_t col, uint8_t (*ev)[128],
+                            uint8_t *rpos, uint8_t *rval) {
     const uint32_t lane = threadIdx.x;
-    for (uint32_t i = lane; i < 512; i += 64) s_exp[i] = gtab[i];
-    for (uint32_t i = lane; i < 256; i += 64) s_log[i] = gtab[512 + i];
-    // byte 1 of a MulTab's first table is c * 1 = c
-    for (uint32_t i = lane; i < (uint32_t)kTwiddleEntries; i += 64) s_tw[i] = (uint8_t)(g_tw[i].w[0] >> 8);
-    const DecPlan *pl = plans + blockIdx.x;
-    const uint32_t col = cols[blockIdx.x];
-    const int dl = pl->deg;
-    __syncthreads();
-    if (dl < 0) {  // zero locator: true, nothing changes
-        if (lane == 0 && status) status[col] = 1;
-        return;
-    }
+    if (dl < 0) return true;
     const uint32_t T = 1u << tl, need = (uint32_t)dl;
     uint32_t cnt = 0;  // roots found so far (uniform)
     for (uint32_t c = 0; c < n / T && cnt < need; ++c) {
         for (uint32_t i = lane; i < T; i += 64) {
-            ev[0][i] = pl->lam[i];
-            ev[1][i] = pl->v[i];
-            ev[2][i] = pl->lp[i];
+            ev[0][i] = lam[i];
+            ev[1][i] = v[i];
+            ev[2][i] = lp[i];
         }
         __syncthreads();
-        // forward FFT of the three polynomials at beta = omega_{cT}
-        // (rs/src/kernel/lut_kernel.rs:11-18, 28-54): levels tl-1 .. 0
         for (int l = (int)tl - 1; l >= 0; --l) {
             for (uint32_t i = lane; i < T / 2; i += 64) {
                 const uint32_t blk = i >> l, pos = i & ((1u << l) - 1u);
                 const uint32_t a = (blk << (l + 1)) + pos, b = a + (1u << l);
                 const uint32_t o = c * T + (blk << (l + 1));
-                const uint8_t t = s_tw[lvl_off(l) + (o >> (l + 1))];
+                const uint8_t tw = t.tw[lvl_off(l) + (o >> (l + 1))];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    const uint8_t xa = ev[k][a] ^ lmul(s_exp, s_log, t, ev[k][b]);
+                    const uint8_t xa = ev[k][a] ^ lmul(t.ex, t.lg, tw, ev[k][b]);
                     ev[k][a] = xa;
                     ev[k][b] ^= xa;
                 }
@@ -222,21 +227,224 @@ __global__ void __launch_bounds__(64) k_decode_tail(uint8_t *__restrict__ rx, ui
             if (root && rank < need) {
                 const uint8_t d = ev[2][off];
                 rpos[rank] = (uint8_t)(c * T + off);
-                rval[rank] = d ? lmul(s_exp, s_log, ev[1][off], s_exp[255 - s_log[d]]) : (uint8_t)0;
+                rval[rank] = d ? lmul(t.ex, t.lg, ev[1][off], t.ex[255 - t.lg[d]]) : (uint8_t)0;
             }
             cnt += (uint32_t)__popcll(m);
         }
         __syncthreads();
     }
-    if (cnt < need) {  // too few roots: false, the column is left as it was
-        if (lane == 0) {
-            if (status) status[col] = 2;
-            atomicAdd(failed, 1u);
-        }
-        return;
-    }
+    if (cnt < need) return false;
     for (uint32_t r = lane; r < need; r += 64) rx[(size_t)rpos[r] * L + col] ^= rval[r];
-    if (lane == 0 && status) status[col] = 1;
+    __syncthreads();
+    return true;
+}
+
+// One wave per listed column (blockIdx.x = list index), plans from the host.
+__global__ void __launch_bounds__(64) k_decode_tail(uint8_t *__restrict__ rx, uint32_t L, uint32_t n, uint32_t tl,
+                                                    const uint32_t *__restrict__ cols,
+                                                    const DecPlan *__restrict__ plans,
+                                                    uint8_t *__restrict__ status, uint32_t *__restrict__ failed,
+                                                    const uint8_t *__restrict__ gtab,
+                                                    const MulTab *__restrict__ g_tw) {
+    __shared__ WaveTabs tabs;
+    __shared__ uint8_t pl[3][128];
+    __shared__ uint8_t ev[3][128];
+    __shared__ uint8_t rpos[256], rval[256];
+    load_wave_tabs(tabs, gtab, g_tw);
+    const DecPlan *p = plans + blockIdx.x;
+    const uint32_t col = cols[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < 128; i += 64) {
+        pl[0][i] = p->lam[i];
+        pl[1][i] = p->v[i];
+        pl[2][i] = p->lp[i];
+    }
+    const int dl = p->deg;
+    __syncthreads();
+    const bool ok = tail_column(tabs, pl[0], pl[1], pl[2], dl, n, tl, rx, L, col, ev, rpos, rval);
+    if (threadIdx.x == 0) {
+        if (status) status[col] = ok ? 1 : 2;
+        if (!ok) atomicAdd(failed, 1u);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The key equation on the device: the same monomial-basis Euclid as the host
+// solver (ars_keyeq.hpp), one wave per column, polynomials as LDS byte arrays
+// of kCap coefficients with the lanes striding over them.  Degrees are
+// wave-uniform values found by ballot.
+
+constexpr int kCap = 264;  // coefficient slots (degrees < 257 and slack)
+
+__device__ __forceinline__ int wave_deg(const uint8_t *p) {
+    int d = -1;
+    for (int s = 0; s < kCap; s += 64) {
+        const int i = s + (int)threadIdx.x;
+        const uint64_t m = __ballot(i < kCap && p[i] != 0);
+        if (m) d = s + 63 - (int)__clzll(m);
+    }
+    return d;
+}
+
+__device__ __forceinline__ void wave_fill(uint8_t *p, uint8_t c0) {
+    for (int i = threadIdx.x; i < kCap; i += 64) p[i] = i == 0 ? c0 : (uint8_t)0;
+}
+
+// r <- r mod b, q <- r div b (deg b = db >= 0).  Returns deg of the new r.
+__device__ int wave_divmod(const WaveTabs &t, uint8_t *r, int dr, const uint8_t *b, int db, uint8_t *q) {
+    wave_fill(q, 0);
+    __syncthreads();
+    const uint8_t binv = t.ex[255 - t.lg[b[db]]];
+    for (int guard = 0; dr >= db && guard < kCap; ++guard) {
+        const int sh = dr - db;
+        const uint8_t f = lmul(t.ex, t.lg, r[dr], binv);
+        __syncthreads();  // every lane has read r[dr]
+        if (threadIdx.x == 0) q[sh] = f;
+        for (int j = threadIdx.x; j <= db; j += 64) r[sh + j] ^= lmul(t.ex, t.lg, f, b[j]);
+        __syncthreads();
+        dr = wave_deg(r);
+    }
+    return dr;
+}
+
+// out = a * b + c (out distinct from a, b, c).  Returns deg out.
+__device__ int wave_muladd(const WaveTabs &t, const uint8_t *a, int da, const uint8_t *b, int db, const uint8_t *c,
+                           uint8_t *out) {
+    for (int i = threadIdx.x; i < kCap; i += 64) {
+        uint8_t acc = c[i];
+        if (da >= 0 && db >= 0 && i <= da + db) {
+            const int j0 = i - db > 0 ? i - db : 0, j1 = i < da ? i : da;
+            for (int j = j0; j <= j1; ++j) acc ^= lmul(t.ex, t.lg, a[j], b[i - j]);
+        }
+        out[i] = acc;
+    }
+    __syncthreads();
+    return wave_deg(out);
+}
+
+// Basis X coefficients of the monomial polynomial r (deg dr, destroyed) into
+// out, by back-substitution on the monic rows X_d of xt (256 x 256).
+__device__ void wave_to_novel(const WaveTabs &t, uint8_t *r, int dr, const uint8_t *__restrict__ xt,
+                              uint8_t *out) {
+    wave_fill(out, 0);
+    __syncthreads();
+    for (int d = dr < 255 ? dr : 255; d >= 0; --d) {
+        const uint8_t a = r[d];
+        __syncthreads();
+        if (a) {
+            if (threadIdx.x == 0) out[d] = a;
+            for (int i = threadIdx.x; i <= d; i += 64) r[i] ^= lmul(t.ex, t.lg, a, xt[d * 256 + i]);
+        }
+        __syncthreads();
+    }
+}
+
+// One wave per listed column, the whole decode of that column: the monomial
+// form of the syndrome, X_T = q_t s + r_t, the remainder sequence of (s, r_t)
+// to the first remainder of degree < T/2 with its cofactors, lambda = u + v q_t,
+// back to the basis X, the derivative (ARS_DEC_FORMAL_DERIVATIVE: formal;
+// otherwise the reference's deriv_lnh), then tail_column.  Grid-stride over
+// the list; its length is read on the device, so no host round trip.
+__global__ void __launch_bounds__(64) k_decode_columns(uint8_t *__restrict__ rx, uint32_t L, uint32_t n, uint32_t tl,
+                                                       const uint32_t *__restrict__ count,
+                                                       const uint32_t *__restrict__ cols,
+                                                       const uint8_t *__restrict__ syn, uint32_t formal,
+                                                       uint8_t *__restrict__ status, uint32_t *__restrict__ failed,
+                                                       const uint8_t *__restrict__ gtab,
+                                                       const MulTab *__restrict__ g_tw,
+                                                       const uint8_t *__restrict__ xt) {
+    enum { S, R, QT, Q, U0, U1, V0, V1, TT, LX, VX, LP, NPOLY };
+    __shared__ WaveTabs tabs;
+    __shared__ uint8_t P[NPOLY][kCap];
+    __shared__ uint8_t sx[128];
+    __shared__ uint8_t ev[3][128];
+    __shared__ uint8_t rpos[256], rval[256];
+    load_wave_tabs(tabs, gtab, g_tw);
+    __syncthreads();
+    const uint32_t lane = threadIdx.x, T = 1u << tl, total = *count;
+    for (uint32_t li = blockIdx.x; li < total; li += gridDim.x) {
+        const uint32_t col = cols[li];
+        for (uint32_t i = lane; i < T; i += 64) sx[i] = syn[(size_t)li * T + i];
+        __syncthreads();
+        // s = sum_j sx[j] X_j
+        for (int i = lane; i < kCap; i += 64) {
+            uint8_t acc = 0;
+            if (i < (int)T)
+                for (uint32_t j = (uint32_t)i; j < T; ++j) acc ^= lmul(tabs.ex, tabs.lg, sx[j], xt[j * 256 + i]);
+            P[S][i] = acc;
+        }
+        __syncthreads();
+        const int ds = wave_deg(P[S]);
+        int dl = -1;
+        if (ds >= 0) {
+            // X_T = q_t s + r_t
+            for (int i = lane; i < kCap; i += 64) P[R][i] = i <= (int)T ? xt[T * 256 + i] : (uint8_t)0;
+            __syncthreads();
+            const int drt = wave_divmod(tabs, P[R], (int)T, P[S], ds, P[QT]);
+            const int dqt = (int)T - ds;
+            // remainder sequence; the buffers rotate through the index variables
+            int z0 = S, z1 = R, dz0 = ds, dz1 = drt;
+            int u0 = U0, u1 = U1, v0 = V0, v1 = V1, tt = TT;
+            wave_fill(P[u0], 1);
+            wave_fill(P[u1], 0);
+            wave_fill(P[v0], 0);
+            wave_fill(P[v1], 1);
+            __syncthreads();
+            int du0 = 0, du1 = -1, dv0 = -1, dv1 = 0;
+            for (int guard = 0; dz1 >= (int)(T / 2) && guard < kCap; ++guard) {
+                const int dr = wave_divmod(tabs, P[z0], dz0, P[z1], dz1, P[Q]);
+                const int dq = dz0 - dz1;
+                const int zr = z0;
+                z0 = z1;
+                dz0 = dz1;
+                z1 = zr;
+                dz1 = dr;
+                const int dnu = wave_muladd(tabs, P[Q], dq, P[u1], du1, P[u0], P[tt]);
+                int f = u0;
+                u0 = u1;
+                du0 = du1;
+                u1 = tt;
+                du1 = dnu;
+                tt = f;
+                const int dnv = wave_muladd(tabs, P[Q], dq, P[v1], dv1, P[v0], P[tt]);
+                f = v0;
+                v0 = v1;
+                dv0 = dv1;
+                v1 = tt;
+                dv1 = dnv;
+                tt = f;
+            }
+            (void)du0;
+            (void)dv0;
+            // lambda = u + v q_t into P[tt]
+            dl = wave_muladd(tabs, P[v1], dv1, P[QT], dqt, P[u1], P[tt]);
+            wave_to_novel(tabs, P[tt], dl, xt, P[LX]);
+            wave_to_novel(tabs, P[v1], dv1, xt, P[VX]);
+            // derivative in the basis X over n coefficients
+            if (formal) {
+                for (uint32_t i = lane; i < n; i += 64) {
+                    uint8_t acc = 0;
+                    for (uint32_t b = 1; b < n; b <<= 1)
+                        if (!(i & b)) acc ^= P[LX][i | b];
+                    P[LP][i] = acc;
+                }
+                __syncthreads();
+            } else {
+                for (uint32_t i = lane; i < n; i += 64) P[LP][i] = P[LX][i];
+                __syncthreads();
+                for (uint32_t half = 1; half < n; half <<= 1) {
+                    for (uint32_t i = lane; i < n; i += 64)
+                        if (!(i & half)) P[LP][i] ^= P[LP][i + half];
+                    __syncthreads();
+                }
+            }
+        }
+        const bool ok = tail_column(tabs, P[LX], P[VX], P[LP], dl, n, tl, rx, L, col, ev, rpos, rval);
+        if (lane == 0) {
+            if (status) status[col] = ok ? 1 : 2;
+            if (!ok) atomicAdd(failed, 1u);
+        }
+        __syncthreads();
+    }
 }
 
 static inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -269,6 +477,16 @@ hipError_t launch_syndrome(int tl, bool al, const uint8_t *rx, uint32_t L, uint3
         default: return hipErrorInvalidValue;
     }
 #undef SY_CASE
+}
+
+hipError_t launch_decode_columns(uint8_t *rx, uint32_t L, uint32_t n, int tl, const uint32_t *count,
+                                 const uint32_t *cols, const uint8_t *syn, bool formal, uint32_t grid,
+                                 uint8_t *status, uint32_t *failed, const uint8_t *gtab, const MulTab *tw,
+                                 const uint8_t *xt, hipStream_t s) {
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_decode_columns, dim3(grid), dim3(64), 0, s, rx, L, n, (uint32_t)tl, count, cols, syn,
+                       formal ? 1u : 0u, status, failed, gtab, tw, xt);
+    return hipGetLastError();
 }
 
 hipError_t launch_decode_tail(uint8_t *rx, uint32_t L, uint32_t n, int tl, const uint32_t *cols,

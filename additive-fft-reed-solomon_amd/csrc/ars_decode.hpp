@@ -18,6 +18,10 @@ struct SynOut {
 
 hipError_t launch_syndrome(int tl, bool aligned, const uint8_t *rx, uint32_t L, uint32_t chunks,
                            const MulTab *tw, const SynOut &so, hipStream_t s);
+hipError_t launch_decode_columns(uint8_t *rx, uint32_t L, uint32_t n, int tl, const uint32_t *count,
+                                 const uint32_t *cols, const uint8_t *syn, bool formal, uint32_t grid,
+                                 uint8_t *status, uint32_t *failed, const uint8_t *gtab, const MulTab *tw,
+                                 const uint8_t *xt, hipStream_t s);
 hipError_t launch_decode_tail(uint8_t *rx, uint32_t L, uint32_t n, int tl, const uint32_t *cols,
                               const DecPlan *plans, uint32_t count, uint8_t *status, uint32_t *failed,
                               const uint8_t *gtab, const MulTab *tw, hipStream_t s);

@@ -1,10 +1,12 @@
 // ars_decode_api.cpp -- ars_decode: sharded error decoding (see ars.h).
 //
 // 1. k_syndrome lists the columns whose syndrome is not zero (device).
-// 2. The host solves the key equation of each listed column on a worker pool
-//    (ars_keyeq.hpp) -- the step the reference runs as HGCD.
-// 3. k_decode_tail runs the root search and error evaluation of every listed
-//    column and applies the corrections in place (device).
+// 2. k_decode_columns solves each listed column's key equation (the step the
+//    reference runs as HGCD) and runs its root search and error evaluation,
+//    one wave per column, reading the list length on the device; the host
+//    waits once, for the counts.
+// With ARS_DEC_HOST_KEY_EQUATION the key equations run on a host worker pool
+// instead (ars_keyeq.hpp) and k_decode_tail does the rest.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -44,6 +46,7 @@ struct DecState {
     ars::DecPlan *plans = nullptr;
     size_t plan_count = 0;
     uint32_t *h_counts = nullptr;
+    uint8_t *xt = nullptr;  // monomial coefficients of X_j, 256 x 256
 };
 
 std::mutex g_states_mu;
@@ -75,6 +78,7 @@ void ars::release_decode_state(ars_ctx *ctx) {
     hipFree(st->dev);
     hipFree(st->plans);
     hipHostFree(st->h_counts);
+    hipFree(st->xt);
     delete st;
 }
 
@@ -83,7 +87,8 @@ extern "C" int ars_decode(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_recei
     if (!ctx || !d_received) return ars::set_error(ARS_E_NULL, "null context or buffer");
     int rc = ars::check_code(n, k, L);
     if (rc) return rc;
-    if (flags & ~ARS_DEC_FORMAL_DERIVATIVE) return ars::set_error(ARS_E_INVAL, "unknown decode flags");
+    if (flags & ~(ARS_DEC_FORMAL_DERIVATIVE | ARS_DEC_HOST_KEY_EQUATION))
+        return ars::set_error(ARS_E_INVAL, "unknown decode flags");
     const uint32_t T = n - k;
     if (!ars::is_pow2(T) || T < 2)
         return ars::set_error(ARS_E_UNSUPPORTED, "error decoding needs T = n - k a power of two >= 2");
@@ -117,10 +122,38 @@ extern "C" int ars_decode(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_recei
     if (d_status && (e = hipMemsetAsync(d_status, 0, L, s)) != hipSuccess) return ars::hip_error(e, "status reset");
 
     const bool al = (L % 4 == 0) && aligned4(d_received);
+    const bool formal = flags & ARS_DEC_FORMAL_DERIVATIVE;
     ars::SynOut so{d_count, d_cols, d_syn};
     if ((e = ars::launch_syndrome(tl, al, d_received, (uint32_t)L, n / T, ctx->d_tw, so, s)) != hipSuccess)
         return ars::hip_error(e, "syndrome launch");
     uint32_t listed = 0, failed = 0;
+    if (!(flags & ARS_DEC_HOST_KEY_EQUATION)) {
+        if (!st->xt) {
+            if ((e = hipMalloc(reinterpret_cast<void **>(&st->xt), 256 * 256)) != hipSuccess)
+                return ars::hip_error(e, "decode basis table");
+            if ((e = hipMemcpy(st->xt, &ars::key_tables().x[0][0], 256 * 256, hipMemcpyHostToDevice)) != hipSuccess) {
+                hipFree(st->xt);
+                st->xt = nullptr;
+                return ars::hip_error(e, "decode basis table copy");
+            }
+        }
+        // enough waves to fill the device; each strides over the list
+        const uint32_t grid = (uint32_t)std::min<size_t>(L, (size_t)std::max(ctx->num_cus, 1) * 32);
+        if ((e = ars::launch_decode_columns(d_received, (uint32_t)L, n, tl, d_count, d_cols, d_syn, formal, grid,
+                                            d_status, d_failed, ctx->d_gtab, ctx->d_tw, st->xt, s)) != hipSuccess)
+            return ars::hip_error(e, "decode launch");
+        if ((e = hipMemcpyAsync(st->h_counts, d_count, 8, hipMemcpyDeviceToHost, s)) != hipSuccess ||
+            (e = hipStreamSynchronize(s)) != hipSuccess)
+            return ars::hip_error(e, "decode result");
+        listed = st->h_counts[0];
+        failed = st->h_counts[1];
+        if (counts) {
+            counts[0] = L - listed;
+            counts[1] = listed - failed;
+            counts[2] = failed;
+        }
+        return failed ? ARS_FALSE : ARS_TRUE;
+    }
     if ((e = hipMemcpyAsync(st->h_counts, d_count, 4, hipMemcpyDeviceToHost, s)) != hipSuccess ||
         (e = hipStreamSynchronize(s)) != hipSuccess)
         return ars::hip_error(e, "syndrome count");
@@ -132,7 +165,6 @@ extern "C" int ars_decode(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_recei
             (e = hipStreamSynchronize(s)) != hipSuccess)
             return ars::hip_error(e, "syndrome copy");
         std::vector<ars::DecPlan> plans(listed);
-        const bool formal = flags & ARS_DEC_FORMAL_DERIVATIVE;
         {
             std::lock_guard<std::mutex> lk(g_solver_mu);
             const size_t per = 16, pieces = (listed + per - 1) / per;

@@ -113,9 +113,9 @@ int ars_recompute_data(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received
  * every byte column of the n x L codeword is one received word, decoded in
  * place exactly as decode_systematic_scalar decodes it: left as it is when
  * its syndrome is zero or when the reference returns false, corrected
- * otherwise.  The syndromes, the root search and the error values run on the
- * GPU; the key equation of each column with a non-zero syndrome is solved on
- * the host, so the call is synchronous on `stream`.
+ * otherwise.  The syndromes, the key equation of each column with a non-zero
+ * syndrome, the root search and the error values all run on the GPU; the call
+ * returns after one synchronisation of `stream` (for the counts).
  *
  * T = n - k must be a power of two >= 2 (the reference's decoder; its HGCD
  * needs t_log >= 1).  flags: 0 reproduces the reference byte for byte,
@@ -123,11 +123,14 @@ int ars_recompute_data(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received
  * (rs/src/poly_arith.rs:91-116) where the formal derivative belongs and are
  * wrong for many patterns of up to T/2 errors; ARS_DEC_FORMAL_DERIVATIVE uses
  * the formal derivative and corrects every pattern of up to T/2 errors.
+ * ARS_DEC_HOST_KEY_EQUATION solves the key equations on the host instead of
+ * the GPU (same results; for cross-checking).
  * d_status (optional, device, L bytes): per column 0 syndrome zero, 1 the
  * reference's true after the key equation, 2 the reference's false.
  * counts (optional, host, 3 values): columns with status 0, 1, 2.
  * Returns ARS_TRUE when no column returned false, ARS_FALSE otherwise. */
 #define ARS_DEC_FORMAL_DERIVATIVE 1u
+#define ARS_DEC_HOST_KEY_EQUATION 2u
 int ars_decode(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_t shard_len,
                uint32_t flags, uint8_t *d_status, uint64_t *counts, void *stream);
 

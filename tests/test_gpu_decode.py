@@ -44,16 +44,17 @@ def make_rx(n, T, L, rng, max_err=None):
     return cw, rx
 
 
-def gpu_decode(ars, n, k, rx, L, formal):
+def gpu_decode(ars, n, k, rx, L, formal, host_keq=False):
     c = ars.Codec(n, k)
     d = dev(rx.reshape(-1))
     st = torch.full((L,), 7, dtype=torch.uint8, device="cuda")
-    ok, counts = c.decode_systematic_sharded(d, L, formal_derivative=formal, status=st)
+    ok, counts = c.decode_systematic_sharded(d, L, formal_derivative=formal, status=st,
+                                             host_key_equation=host_keq)
     return ok, counts, host(d).reshape(n, L), host(st)
 
 
-def check_against_oracle(ars, n, T, rx, L, formal):
-    got_ok, counts, got, st = gpu_decode(ars, n, n - T, rx, L, formal)
+def check_against_oracle(ars, n, T, rx, L, formal, host_keq=False):
+    got_ok, counts, got, st = gpu_decode(ars, n, n - T, rx, L, formal, host_keq)
     exp = rx.reshape(-1).copy()
     rc, est = O.decode_sharded(n, T, exp, L, formal=formal)
     assert rc in (0, 1), "the reference would panic on this input"
@@ -72,7 +73,8 @@ def test_decode_matches_oracle(ars, n, T, L):
     rng = np.random.default_rng(n * 1009 + T * 17 + L)
     _, rx = make_rx(n, T, L, rng)
     for formal in (False, True):
-        check_against_oracle(ars, n, T, rx, L, formal)
+        for host_keq in (False, True):  # key equations on the GPU, then on the host
+            check_against_oracle(ars, n, T, rx, L, formal, host_keq)
 
 
 @pytest.mark.parametrize("n,T", [(16, 8), (64, 32), (256, 128), (256, 16), (128, 4)])
@@ -115,9 +117,10 @@ def test_decode_whole_shard_corruption(ars):
     rx = cw.copy()
     bad = rng.permutation(n)[:5]
     rx[bad] ^= rng.integers(1, 256, (5, L), dtype=np.uint8)
-    ok, counts, got, st = gpu_decode(ars, n, n - T, rx, L, True)
-    assert ok and counts == (0, L, 0) and np.array_equal(got, cw)
-    check_against_oracle(ars, n, T, rx[:, :512].copy(), 512, False)
+    for host_keq in (False, True):
+        ok, counts, got, st = gpu_decode(ars, n, n - T, rx, L, True, host_keq)
+        assert ok and counts == (0, L, 0) and np.array_equal(got, cw)
+        check_against_oracle(ars, n, T, rx[:, :512].copy(), 512, False, host_keq)
 
 
 def test_decode_unaligned_and_clean(ars):
@@ -147,7 +150,7 @@ def test_decode_rejects_unsupported_codes(ars):
             ctx.decode(n, k, d.data_ptr(), 16)
         assert ei.value.code == ars.ARS_E_UNSUPPORTED
     with pytest.raises(ars.ArsError) as ei:
-        ctx.decode(8, 4, d.data_ptr(), 16, flags=4)
+        ctx.decode(8, 4, d.data_ptr(), 16, flags=8)
     assert ei.value.code == ars.ARS_E_INVAL
 
 
