@@ -160,11 +160,11 @@ static inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a +
 template <int TL>
 constexpr int tile_pq() { return TL >= 8 ? 32 : 64; }
 
-// Tile variants the autotuner chooses among: column quads per workgroup (PQ)
+// Tile variants the autotuner chooses among: column quads per workgroup (PQ: 16, 32, 64)
 // and LDS buffers (NBUF: 2 = one barrier per exchange, 1 = half the LDS).
 static bool variant_ok(int tl, int pq, int nbuf) {
     if (tl <= 4) return true;
-    if (pq != 32 && pq != 64) return false;
+    if (pq != 16 && pq != 32 && pq != 64) return false;
     if (nbuf != 1 && nbuf != 2) return false;
     return ((1 << (tl - 4)) * pq) <= 1024;
 }
@@ -183,7 +183,7 @@ static hipError_t enc_one(const uint8_t *msg, uint8_t *par, uint32_t L, uint32_t
                            dim3(Tile<TL, PQ_>::THREADS), 0, s, msg, par, L, chunks, tw);             \
         return hipGetLastError();                                                                    \
     }
-        ENC_V(32, 1) ENC_V(32, 2) ENC_V(64, 1) ENC_V(64, 2)
+        ENC_V(16, 1) ENC_V(16, 2) ENC_V(32, 1) ENC_V(32, 2) ENC_V(64, 1) ENC_V(64, 2)
 #undef ENC_V
         return hipErrorInvalidValue;
     }
@@ -266,7 +266,7 @@ static hipError_t rec_one(uint8_t *data, uint32_t L, const RecPrep *prep, const 
                            dim3(Tile<NL, PQ_>::THREADS), 0, s, data, L, prep, tw);                   \
         return hipGetLastError();                                                                    \
     }
-        REC_V(32, 1) REC_V(32, 2) REC_V(64, 1) REC_V(64, 2)
+        REC_V(16, 1) REC_V(16, 2) REC_V(32, 1) REC_V(32, 2) REC_V(64, 1) REC_V(64, 2)
 #undef REC_V
         return hipErrorInvalidValue;
     }

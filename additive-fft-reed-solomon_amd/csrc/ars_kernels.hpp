@@ -22,7 +22,7 @@ struct PosArg {
 constexpr int kGtabBytes = 1024;
 
 // pq / nbuf select a tile variant for transforms of 32+ points (0 = default):
-// pq = column quads per workgroup (32 or 64), nbuf = LDS exchange buffers (1 or 2).
+// pq = column quads per workgroup (16, 32 or 64), nbuf = LDS exchange buffers (1 or 2).
 hipError_t launch_encode_ref(int tl, bool aligned, bool accumulate, const uint8_t *msg, uint8_t *par,
                              uint32_t L, uint32_t chunks, const MulTab *tw, hipStream_t s,
                              int pq = 0, int nbuf = 0);

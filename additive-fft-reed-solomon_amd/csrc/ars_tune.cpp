@@ -12,7 +12,7 @@ namespace {
 struct Variant {
     int pq, nbuf;
 };
-const Variant kVariants[] = {{32, 1}, {32, 2}, {64, 1}, {64, 2}};
+const Variant kVariants[] = {{16, 1}, {16, 2}, {32, 1}, {32, 2}, {64, 1}, {64, 2}};
 constexpr int kReps = 5;
 
 // Mean time of `reps` launches of fn after one warm-up, or a negative value
@@ -36,7 +36,7 @@ extern "C" {
 int ars_tune_set(ars_ctx *ctx, uint32_t op, uint32_t log2_size, uint32_t quads, uint32_t buffers) {
     if (!ctx) return ars::set_error(ARS_E_NULL, "null context");
     if ((op != ARS_OP_ENCODE && op != ARS_OP_RECOVER) || log2_size > 8 ||
-        (quads && quads != 32 && quads != 64) || (buffers && buffers != 1 && buffers != 2))
+        (quads && quads != 16 && quads != 32 && quads != 64) || (buffers && buffers != 1 && buffers != 2))
         return ars::set_error(ARS_E_INVAL, "bad tuning entry");
     const int o = op == ARS_OP_ENCODE ? 0 : 1;
     ctx->tune_pq[o][log2_size] = (uint8_t)quads;

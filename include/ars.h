@@ -246,7 +246,7 @@ int ars_server_replay(ars_server *s, const ars_request *reqs, const double *arri
 
 /* ------------------------------------------------------------------------
  * Autotuner.  For transforms of 32+ points the fused kernels come in tile
- * variants: column quads per workgroup (32 or 64) and LDS exchange buffers
+ * variants: column quads per workgroup (16, 32 or 64) and LDS exchange buffers
  * (1: half the LDS, extra barriers; 2: one barrier per exchange).
  * ars_autotune times every variant for one (n, k, shard_len) on scratch
  * buffers and keeps the fastest in the context for that operation and
