@@ -321,20 +321,42 @@ __device__ int wave_muladd(const WaveTabs &t, const uint8_t *a, int da, const ui
     return wave_deg(out);
 }
 
+// The basis table xl (256 x 256) holds the monomial coefficients of X_j in
+// the log domain: xl[j][i] = log(coefficient of x^i in X_j), kLogZero for 0.
+constexpr uint8_t kLogZero = 255;
+
+// a * X coefficient, a given by its log la (kLogZero for 0).
+__device__ __forceinline__ uint8_t lmul_log(const WaveTabs &t, uint8_t la, uint8_t lx) {
+    return (la != kLogZero && lx != kLogZero) ? t.ex[la + lx] : (uint8_t)0;
+}
+
 // Basis X coefficients of the monomial polynomial r (deg dr, destroyed) into
-// out, by back-substitution on the monic rows X_d of xt (256 x 256).
-__device__ void wave_to_novel(const WaveTabs &t, uint8_t *r, int dr, const uint8_t *__restrict__ xt,
+// out, by back-substitution on the monic rows X_d.  Row d has zeros above
+// column d, so every lane updates its four slots of r unconditionally, and
+// row d - 1 is loaded while row d is applied.
+__device__ void wave_to_novel(const WaveTabs &t, uint8_t *r, int dr, const uint8_t *__restrict__ xl,
                               uint8_t *out) {
+    const int lane = threadIdx.x;
     wave_fill(out, 0);
+    int d = dr < 255 ? dr : 255;
+    uint8_t cur[4], nxt[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cur[k] = d >= 0 ? xl[d * 256 + lane + 64 * k] : kLogZero;
     __syncthreads();
-    for (int d = dr < 255 ? dr : 255; d >= 0; --d) {
+    for (; d >= 0; --d) {
         const uint8_t a = r[d];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) nxt[k] = d > 0 ? xl[(d - 1) * 256 + lane + 64 * k] : kLogZero;
         __syncthreads();
         if (a) {
-            if (threadIdx.x == 0) out[d] = a;
-            for (int i = threadIdx.x; i <= d; i += 64) r[i] ^= lmul(t.ex, t.lg, a, xt[d * 256 + i]);
+            const uint8_t la = t.lg[a];
+            if (lane == 0) out[d] = a;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) r[lane + 64 * k] ^= lmul_log(t, la, cur[k]);
         }
         __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cur[k] = nxt[k];
     }
 }
 
@@ -352,6 +374,7 @@ __global__ void __launch_bounds__(64) k_decode_columns(uint8_t *__restrict__ rx,
                                                        const uint8_t *__restrict__ gtab,
                                                        const MulTab *__restrict__ g_tw,
                                                        const uint8_t *__restrict__ xt) {
+    // xt: the basis table in the log domain (see wave_to_novel)
     enum { S, R, QT, Q, U0, U1, V0, V1, TT, LX, VX, LP, NPOLY };
     __shared__ WaveTabs tabs;
     __shared__ uint8_t P[NPOLY][kCap];
@@ -363,13 +386,19 @@ __global__ void __launch_bounds__(64) k_decode_columns(uint8_t *__restrict__ rx,
     const uint32_t lane = threadIdx.x, T = 1u << tl, total = *count;
     for (uint32_t li = blockIdx.x; li < total; li += gridDim.x) {
         const uint32_t col = cols[li];
-        for (uint32_t i = lane; i < T; i += 64) sx[i] = syn[(size_t)li * T + i];
+        for (uint32_t i = lane; i < T; i += 64) {
+            const uint8_t v = syn[(size_t)li * T + i];
+            sx[i] = v ? tabs.lg[v] : kLogZero;  // logs of the syndrome coefficients
+        }
         __syncthreads();
-        // s = sum_j sx[j] X_j
+        // s = sum_j sx[j] X_j (row j is zero above column j, so the loop runs
+        // the same trip count on every lane)
         for (int i = lane; i < kCap; i += 64) {
             uint8_t acc = 0;
-            if (i < (int)T)
-                for (uint32_t j = (uint32_t)i; j < T; ++j) acc ^= lmul(tabs.ex, tabs.lg, sx[j], xt[j * 256 + i]);
+            if (i < (int)T) {
+#pragma unroll 8
+                for (uint32_t j = 0; j < T; ++j) acc ^= lmul_log(tabs, sx[j], xt[j * 256 + i]);
+            }
             P[S][i] = acc;
         }
         __syncthreads();
@@ -377,7 +406,10 @@ __global__ void __launch_bounds__(64) k_decode_columns(uint8_t *__restrict__ rx,
         int dl = -1;
         if (ds >= 0) {
             // X_T = q_t s + r_t
-            for (int i = lane; i < kCap; i += 64) P[R][i] = i <= (int)T ? xt[T * 256 + i] : (uint8_t)0;
+            for (int i = lane; i < kCap; i += 64) {
+                const uint8_t l = i <= (int)T ? xt[T * 256 + i] : kLogZero;
+                P[R][i] = l == kLogZero ? (uint8_t)0 : tabs.ex[l];
+            }
             __syncthreads();
             const int drt = wave_divmod(tabs, P[R], (int)T, P[S], ds, P[QT]);
             const int dqt = (int)T - ds;

@@ -46,7 +46,7 @@ struct DecState {
     ars::DecPlan *plans = nullptr;
     size_t plan_count = 0;
     uint32_t *h_counts = nullptr;
-    uint8_t *xt = nullptr;  // monomial coefficients of X_j, 256 x 256
+    uint8_t *xt = nullptr;  // monomial coefficients of X_j, 256 x 256, log domain
 };
 
 std::mutex g_states_mu;
@@ -129,9 +129,15 @@ extern "C" int ars_decode(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_recei
     uint32_t listed = 0, failed = 0;
     if (!(flags & ARS_DEC_HOST_KEY_EQUATION)) {
         if (!st->xt) {
+            // monomial coefficients of X_j in the log domain, 255 for zero
+            std::vector<uint8_t> xl(256 * 256);
+            const ars::KeyTables &kt = ars::key_tables();
+            const ars::GfTables &g = ars::gf();
+            for (int j = 0; j < 256; ++j)
+                for (int i = 0; i < 256; ++i) xl[j * 256 + i] = kt.x[j][i] ? g.log[kt.x[j][i]] : (uint8_t)255;
             if ((e = hipMalloc(reinterpret_cast<void **>(&st->xt), 256 * 256)) != hipSuccess)
                 return ars::hip_error(e, "decode basis table");
-            if ((e = hipMemcpy(st->xt, &ars::key_tables().x[0][0], 256 * 256, hipMemcpyHostToDevice)) != hipSuccess) {
+            if ((e = hipMemcpy(st->xt, xl.data(), 256 * 256, hipMemcpyHostToDevice)) != hipSuccess) {
                 hipFree(st->xt);
                 st->xt = nullptr;
                 return ars::hip_error(e, "decode basis table copy");
