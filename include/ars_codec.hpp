@@ -8,6 +8,8 @@
 //       (rs/src/codec.rs:241-265)
 //   recover_erasure_shards(&self, received, workspace, shard_len, positions) -> bool
 //       (rs/src/codec.rs:479-569)
+// plus recompute_data_from_parity_sharded (rs/src/codec.rs:321-340) and a
+// sharded form of decode_systematic_scalar (rs/src/codec.rs:341-420).
 // `Rs<N, T>` here keeps those names, argument meaning and return values on
 // device buffers.  The reference panics on out-of-range positions; this
 // mirror throws ars::Error, as it does for every negative ARS_E_* code.
@@ -72,6 +74,19 @@ class Rs {
                                             std::size_t shard_len) const {
         static_assert(N == 2 * T, "recompute_data_from_parity_sharded needs N = 2T");
         check(ars_recompute_data(ctx_, N, N - T, received, shard_len, stream_));
+    }
+
+    // Every byte column decoded as Codec::decode_systematic_scalar
+    // (rs/src/codec.rs:341-420) decodes one received word, in place on the
+    // device buffer `received` (N*shard_len bytes).  True iff no column
+    // returned false.  formal_derivative: evaluate errors with the formal
+    // derivative instead of the reference's poly::deriv_lnh.  Needs T >= 2.
+    bool decode_systematic_sharded(uint8_t *received, std::size_t shard_len, bool formal_derivative = false,
+                                   uint64_t *counts = nullptr) const {
+        static_assert(T >= 2, "error decoding needs T >= 2");
+        return check(ars_decode(ctx_, N, N - T, received, shard_len,
+                                formal_derivative ? ARS_DEC_FORMAL_DERIVATIVE : 0u, nullptr, counts, stream_)) ==
+               ARS_TRUE;
     }
 
   private:

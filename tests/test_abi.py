@@ -59,11 +59,18 @@ def test_cpp_mirror_compiles_and_reports_errors(tmp_path):
     src.write_text(r'''
 #include "ars_codec.hpp"
 #include <cstdio>
+#include <cstdlib>
 int main() {
     if (ars_encode(nullptr, 64, 32, nullptr, nullptr, 64, 0, nullptr) != ARS_E_NULL) return 2;
     try {
         ars::Rs<64, 32> rs;
         std::puts("context ok");
+        if (std::getenv("ARS_MIRROR_NEVER_SET")) {  // instantiate every method, run none
+            rs.encode_systematic_sharded(nullptr, nullptr, nullptr, 0);
+            rs.recover_erasure_shards(nullptr, nullptr, 0, nullptr, 0);
+            rs.recompute_data_from_parity_sharded(nullptr, nullptr, 0);
+            rs.decode_systematic_sharded(nullptr, 0, true);
+        }
     } catch (const ars::Error &e) {
         std::printf("error %d\n", e.code);
         if (e.code != ARS_E_HIP) return 3;
@@ -84,3 +91,7 @@ int main() {
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(bad)],
                        capture_output=True, text=True)
     assert r.returncode != 0 and "power of two" in r.stderr
+    bad.write_text('#include "ars_codec.hpp"\nint main(){ ars::Rs<4, 1> rs; rs.decode_systematic_sharded(nullptr, 0); }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(bad)],
+                       capture_output=True, text=True)
+    assert r.returncode != 0 and "T >= 2" in r.stderr
