@@ -149,6 +149,25 @@ void scatter(WorkPool *pool, uint8_t *dst, const uint8_t *src, size_t L, size_t 
     copy_rows(pool, dst, const_cast<uint8_t *>(src), L, c0, w, rows, false);
 }
 
+// Waits for chunks a previous call left in flight (it returned an error
+// midway) and forgets them: their output belonged to that call's buffers.
+void discard_pending(Pipe *p) {
+    for (Slot &s : p->slots) {
+        if (!s.pending) continue;
+        hipEventSynchronize(s.done);
+        s.pending = false;
+    }
+}
+
+// Runs a pipeline body; whatever it returns, no chunk stays pending.
+template <class F>
+int run_pipe(Pipe *p, F &&body) {
+    discard_pending(p);
+    const int rc = body();
+    if (rc < 0) discard_pending(p);
+    return rc;
+}
+
 }  // namespace
 
 void ars::release_host_pipe(ars_ctx *ctx) {
@@ -193,39 +212,41 @@ int ars_encode_host(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *message
     if (!p) return ars::set_error(ARS_E_NOMEM, "host pipeline allocation failed");
     std::lock_guard<std::mutex> lk(p->mu);
     if ((rc = init_pipe(ctx, p))) return rc;
-    const bool acc = flags & ARS_ENC_ACCUMULATE;
-    const size_t W = chunk_width(n, L);
-    int si = 0;
-    for (size_t c0 = 0; c0 < L; c0 += W, si = (si + 1) % kSlots) {
-        Slot &s = p->slots[si];
-        const size_t w = std::min(W, L - c0);
-        if (s.pending) {  // retire this slot's previous chunk
+    return run_pipe(p, [&]() -> int {
+        const bool acc = flags & ARS_ENC_ACCUMULATE;
+        const size_t W = chunk_width(n, L);
+        int si = 0;
+        for (size_t c0 = 0; c0 < L; c0 += W, si = (si + 1) % kSlots) {
+            Slot &s = p->slots[si];
+            const size_t w = std::min(W, L - c0);
+            if (s.pending) {  // retire this slot's previous chunk
+                if ((e = hipEventSynchronize(s.done)) != hipSuccess) return ars::hip_error(e, "host pipeline");
+                scatter(p->pool, parity, s.h_out, L, s.c0, s.w, T);
+                s.pending = false;
+            }
+            uint8_t *d_par = s.dev, *d_msg = s.dev + (size_t)T * w;
+            gather(p->pool, s.h_in + (size_t)T * w, message, L, c0, w, k);
+            if (acc) gather(p->pool, s.h_in, parity, L, c0, w, T);
+            const size_t in_off = acc ? 0 : (size_t)T * w;
+            if ((e = hipMemcpyAsync(s.dev + in_off, s.h_in + in_off, (size_t)n * w - in_off, hipMemcpyHostToDevice,
+                                    s.stream)) != hipSuccess)
+                return ars::hip_error(e, "host pipeline copy-in");
+            if ((rc = ars_encode(ctx, n, k, d_msg, d_par, w, flags, s.stream)) < 0) return rc;
+            if ((e = hipMemcpyAsync(s.h_out, d_par, (size_t)T * w, hipMemcpyDeviceToHost, s.stream)) != hipSuccess ||
+                (e = hipEventRecord(s.done, s.stream)) != hipSuccess)
+                return ars::hip_error(e, "host pipeline copy-out");
+            s.pending = true;
+            s.c0 = c0;
+            s.w = w;
+        }
+        for (Slot &s : p->slots) {
+            if (!s.pending) continue;
             if ((e = hipEventSynchronize(s.done)) != hipSuccess) return ars::hip_error(e, "host pipeline");
             scatter(p->pool, parity, s.h_out, L, s.c0, s.w, T);
             s.pending = false;
         }
-        uint8_t *d_par = s.dev, *d_msg = s.dev + (size_t)T * w;
-        gather(p->pool, s.h_in + (size_t)T * w, message, L, c0, w, k);
-        if (acc) gather(p->pool, s.h_in, parity, L, c0, w, T);
-        const size_t in_off = acc ? 0 : (size_t)T * w;
-        if ((e = hipMemcpyAsync(s.dev + in_off, s.h_in + in_off, (size_t)n * w - in_off, hipMemcpyHostToDevice,
-                                s.stream)) != hipSuccess)
-            return ars::hip_error(e, "host pipeline copy-in");
-        if ((rc = ars_encode(ctx, n, k, d_msg, d_par, w, flags, s.stream)) < 0) return rc;
-        if ((e = hipMemcpyAsync(s.h_out, d_par, (size_t)T * w, hipMemcpyDeviceToHost, s.stream)) != hipSuccess ||
-            (e = hipEventRecord(s.done, s.stream)) != hipSuccess)
-            return ars::hip_error(e, "host pipeline copy-out");
-        s.pending = true;
-        s.c0 = c0;
-        s.w = w;
-    }
-    for (Slot &s : p->slots) {
-        if (!s.pending) continue;
-        if ((e = hipEventSynchronize(s.done)) != hipSuccess) return ars::hip_error(e, "host pipeline");
-        scatter(p->pool, parity, s.h_out, L, s.c0, s.w, T);
-        s.pending = false;
-    }
-    return ARS_OK;
+        return ARS_OK;
+    });
 }
 
 int ars_recover_host(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *received, size_t L,
@@ -241,43 +262,45 @@ int ars_recover_host(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *received, si
     if (!p) return ars::set_error(ARS_E_NOMEM, "host pipeline allocation failed");
     std::lock_guard<std::mutex> lk(p->mu);
     if ((rc = init_pipe(ctx, p))) return rc;
-    // one erasure plan for every chunk (validates the positions)
-    if ((rc = ars_recover_plan(ctx, n, k, positions, e_cnt, p->plan, p->slots[0].stream)) < 0) return rc;
-    if ((e = hipStreamSynchronize(p->slots[0].stream)) != hipSuccess) return ars::hip_error(e, "recover plan");
-    std::vector<uint8_t> er(positions, positions + e_cnt);
-    RowList rows{};
-    std::memcpy(rows.r, positions, e_cnt);
-    const size_t W = chunk_width(n, L);
-    auto retire = [&](Slot &s) -> int {
-        hipError_t ee = hipEventSynchronize(s.done);
-        if (ee != hipSuccess) return ars::hip_error(ee, "host pipeline");
-        copy_packed(p->pool, received, s.h_out, L, s.c0, s.w, er.data(), e_cnt);
-        s.pending = false;
-        return ARS_OK;
-    };
-    int si = 0;
-    for (size_t c0 = 0; c0 < L; c0 += W, si = (si + 1) % kSlots) {
-        Slot &s = p->slots[si];
-        const size_t w = std::min(W, L - c0);
-        if (s.pending && (rc = retire(s))) return rc;
-        gather(p->pool, s.h_in, received, L, c0, w, n);
-        if ((e = hipMemcpyAsync(s.dev, s.h_in, (size_t)n * w, hipMemcpyHostToDevice, s.stream)) != hipSuccess)
-            return ars::hip_error(e, "host pipeline copy-in");
-        if ((rc = ars_recover_with_plan(ctx, n, k, s.dev, w, p->plan, s.stream)) < 0) return rc;
-        uint8_t *packed = s.dev + (size_t)n * w;
-        k_pack_rows<<<dim3((unsigned)((w + 4095) / 4096), e_cnt), 256, 0, s.stream>>>(s.dev, packed, rows,
-                                                                                    (uint32_t)w);
-        if ((e = hipGetLastError()) != hipSuccess) return ars::hip_error(e, "pack launch");
-        if ((e = hipMemcpyAsync(s.h_out, packed, (size_t)e_cnt * w, hipMemcpyDeviceToHost, s.stream)) != hipSuccess)
-            return ars::hip_error(e, "host pipeline copy-out");
-        if ((e = hipEventRecord(s.done, s.stream)) != hipSuccess) return ars::hip_error(e, "hipEventRecord");
-        s.pending = true;
-        s.c0 = c0;
-        s.w = w;
-    }
-    for (Slot &s : p->slots)
-        if (s.pending && (rc = retire(s))) return rc;
-    return ARS_TRUE;
+    return run_pipe(p, [&]() -> int {
+        // one erasure plan for every chunk (validates the positions)
+        if ((rc = ars_recover_plan(ctx, n, k, positions, e_cnt, p->plan, p->slots[0].stream)) < 0) return rc;
+        if ((e = hipStreamSynchronize(p->slots[0].stream)) != hipSuccess) return ars::hip_error(e, "recover plan");
+        std::vector<uint8_t> er(positions, positions + e_cnt);
+        RowList rows{};
+        std::memcpy(rows.r, positions, e_cnt);
+        const size_t W = chunk_width(n, L);
+        auto retire = [&](Slot &s) -> int {
+            hipError_t ee = hipEventSynchronize(s.done);
+            if (ee != hipSuccess) return ars::hip_error(ee, "host pipeline");
+            copy_packed(p->pool, received, s.h_out, L, s.c0, s.w, er.data(), e_cnt);
+            s.pending = false;
+            return ARS_OK;
+        };
+        int si = 0;
+        for (size_t c0 = 0; c0 < L; c0 += W, si = (si + 1) % kSlots) {
+            Slot &s = p->slots[si];
+            const size_t w = std::min(W, L - c0);
+            if (s.pending && (rc = retire(s))) return rc;
+            gather(p->pool, s.h_in, received, L, c0, w, n);
+            if ((e = hipMemcpyAsync(s.dev, s.h_in, (size_t)n * w, hipMemcpyHostToDevice, s.stream)) != hipSuccess)
+                return ars::hip_error(e, "host pipeline copy-in");
+            if ((rc = ars_recover_with_plan(ctx, n, k, s.dev, w, p->plan, s.stream)) < 0) return rc;
+            uint8_t *packed = s.dev + (size_t)n * w;
+            k_pack_rows<<<dim3((unsigned)((w + 4095) / 4096), e_cnt), 256, 0, s.stream>>>(s.dev, packed, rows,
+                                                                                        (uint32_t)w);
+            if ((e = hipGetLastError()) != hipSuccess) return ars::hip_error(e, "pack launch");
+            if ((e = hipMemcpyAsync(s.h_out, packed, (size_t)e_cnt * w, hipMemcpyDeviceToHost, s.stream)) != hipSuccess)
+                return ars::hip_error(e, "host pipeline copy-out");
+            if ((e = hipEventRecord(s.done, s.stream)) != hipSuccess) return ars::hip_error(e, "hipEventRecord");
+            s.pending = true;
+            s.c0 = c0;
+            s.w = w;
+        }
+        for (Slot &s : p->slots)
+            if (s.pending && (rc = retire(s))) return rc;
+        return ARS_TRUE;
+    });
 }
 
 }  // extern "C"

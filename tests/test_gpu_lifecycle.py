@@ -82,3 +82,24 @@ def test_two_threads_two_contexts(ars):
         t.join(timeout=300)
     assert set(results) == {0, 1}
     assert not results[0] and not results[1], results
+
+
+def test_host_slices_after_a_rejected_call(ars):
+    """A host-slice call that fails (duplicate positions) leaves nothing in
+    flight that a later call on the same context could pick up."""
+    ctx = ars.Context(0)
+    try:
+        n, k, L = 64, 32, 300000  # several pipeline chunks
+        T = n - k
+        rng = np.random.default_rng(77)
+        msg = rng.integers(0, 256, k * L, dtype=np.uint8)
+        cw = np.concatenate([O.encode(n, T, msg, L), msg])
+        rx = cw.copy()
+        dup = np.array([3, 5, 3], np.uint8)
+        rc = ars.lib().ars_recover_host(ctx.handle, n, k, rx.ctypes.data, L, dup.ctypes.data, len(dup))
+        assert rc == ars.ARS_E_DUPLICATE
+        for seed in range(2):
+            bad = work(ars, ctx, n, k, L // 3 + seed, 200 + seed)
+            assert not bad, bad
+    finally:
+        ctx.close()
