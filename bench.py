@@ -139,22 +139,46 @@ def measured_traffic(kernel):
     return rec["traffic_bytes"] if rec else None
 
 
-def valu_roofline(kernel, seconds):
-    """Secondary roofline: VALU lane-ops per launch of `kernel` (committed
-    SQ_INSTS_VALU x waves x 64, profiles/valu_256x128_1MiB.json) over the
-    launch time measured now, against 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz."""
+def probe_valu_rates():
+    """Live VALU issue rates (wave64 instructions / s) from libars_probe.so:
+    independent v_xor_b32 chains, v_perm_b32 chains, and the fused kernels'
+    butterfly loop on registers.  None if the probe library is missing."""
+    import ctypes
+    path = os.path.join(ROOT, "additive-fft-reed-solomon_amd", "libars_probe.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    lib.ars_probe_valu_rates.restype = ctypes.c_int
+    lib.ars_probe_valu_rates.argtypes = [ctypes.POINTER(ctypes.c_double)]
+    out = (ctypes.c_double * 3)()
+    if lib.ars_probe_valu_rates(out) != 0:
+        return None
+    return {"simple": out[0], "perm": out[1], "butterfly_loop": out[2]}
+
+
+def valu_roofline(kernel, seconds, rates):
+    """Secondary roofline: the kernel's VALU instruction rate (committed
+    SQ_INSTS_VALU per wave x waves, profiles/valu_256x128_1MiB.json, over the
+    launch time measured now) against the attainable rate for its instruction
+    mix: the live simple-op and v_perm issue rates weighted by the kernel's
+    v_perm share."""
     try:
         with open(os.path.join(ROOT, "profiles", "valu_256x128_1MiB.json")) as f:
-            peak = json.load(f)["valu_peak_lane_ops_per_s"]
+            perm = json.load(f)["perm_fraction"]
     except (OSError, ValueError, KeyError):
         return None
     name, rec = _profile_entry("valu_256x128_1MiB.json", kernel)
-    if rec is None:
+    if rec is None or rates is None:
         return None
-    achieved = rec["lane_ops_per_launch"] / seconds
-    return {"bound": "valu", "kernel": name, "achieved": round(achieved / 1e12, 2), "peak": round(peak / 1e12, 2),
-            "unit": "T lane-ops/s", "frac": round(achieved / peak, 4),
-            "lane_ops_per_launch": rec["lane_ops_per_launch"]}
+    fam = next((v["value"] for k, v in perm.items() if name.startswith(k)), None)
+    if fam is None:
+        return None
+    achieved = rec["valu_instr_per_wave"] * rec["waves"] / seconds
+    peak = 1.0 / (fam / rates["perm"] + (1.0 - fam) / rates["simple"])
+    return {"bound": "valu", "kernel": name, "achieved": round(achieved / 1e12, 3), "peak": round(peak / 1e12, 3),
+            "unit": "T wave64-instr/s", "frac": round(achieved / peak, 4), "perm_fraction": fam,
+            "butterfly_loop_frac": round(rates["butterfly_loop"] / peak, 4),
+            "peak_from": "live v_xor_b32 and v_perm_b32 issue rates weighted by the kernel's v_perm share"}
 
 
 def bench_batched(ars, torch, np, stream, count=4096, seed=44, reps=3):
@@ -499,6 +523,7 @@ def main():
         dom, dom_bytes, dom_s = enc_name, n * L, s_enc
     achieved = dom_bytes / dom_s / 1e9
     traffic = measured_traffic(dom)
+    valu_rates = probe_valu_rates()
 
     out = {
         "metric": METRIC,
@@ -519,13 +544,14 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                      "traffic": traffic, "algorithmic_bytes": dom_bytes},
-        "valu_roofline": valu_roofline(dom, dom_s),
+        "valu_roofline": valu_roofline(dom, dom_s, valu_rates),
+        "valu_issue_rates": {k: round(v / 1e12, 3) for k, v in valu_rates.items()} if valu_rates else None,
         "encode": {"GBps": round(enc_gbps, 1), "us": round(s_enc * 1e6, 2),
                    "frac_hbm": round(enc_gbps / HBM_PEAK_GBPS, 4), "bytes": n * L,
-                   "valu_roofline": valu_roofline(enc_name, s_enc)},
+                   "valu_roofline": valu_roofline(enc_name, s_enc, valu_rates)},
         "recover": {"GBps": round(rec_gbps, 1), "us": round(s_rec * 1e6, 2),
                     "frac_hbm": round(rec_gbps / HBM_PEAK_GBPS, 4), "bytes": (n + e) * L,
-                    "valu_roofline": valu_roofline(rec_name, s_rec)},
+                    "valu_roofline": valu_roofline(rec_name, s_rec, valu_rates)},
         "autotune": {"encode": tune_enc, "recover": tune_rec},
         "copy_reference": {"GBps": round(copy_gbps, 1), "bytes_read_plus_written": 2 * n * L,
                            "frac_hbm": round(copy_gbps / HBM_PEAK_GBPS, 4)},
