@@ -20,6 +20,7 @@ struct ars_batch {
     uint8_t *h_stage = nullptr;  // pinned mirror of the arena's descriptor part
     size_t stage_cap = 0;
     hipEvent_t upload_done = nullptr;
+    hipEvent_t run_done = nullptr;  // the last run's kernels (any stream) have finished reading the arena
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t exec = nullptr;
     uint32_t grid = 0, prep_grid = 0, blocks_per_cu = 0;
@@ -103,6 +104,10 @@ int ensure_capacity(ars_batch *b, size_t arena_bytes, size_t stage_bytes) {
 
 }  // namespace
 
+namespace ars {
+ars_ctx *batch_context(const ars_batch *b) { return b->ctx; }
+}  // namespace ars
+
 extern "C" {
 
 int ars_batch_create(ars_ctx *ctx, uint32_t max_jobs, ars_batch **out) {
@@ -115,6 +120,7 @@ int ars_batch_create(ars_ctx *ctx, uint32_t max_jobs, ars_batch **out) {
     b->ctx = ctx;
     b->max_jobs = max_jobs ? max_jobs : 1;
     if ((e = hipEventCreateWithFlags(&b->upload_done, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&b->run_done, hipEventDisableTiming)) != hipSuccess ||
         (e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking)) != hipSuccess) {
         ars_batch_destroy(b);
         return ars::hip_error(e, "batch events/streams");
@@ -134,6 +140,7 @@ int ars_batch_destroy(ars_batch *b) {
     destroy_graph(b);
     if (b->cap_stream) hipStreamDestroy(b->cap_stream);
     if (b->upload_done) hipEventDestroy(b->upload_done);
+    if (b->run_done) hipEventDestroy(b->run_done);
     hipFree(b->d_arena);
     hipHostFree(b->h_stage);
     delete b;
@@ -318,6 +325,9 @@ int ars_batch_run(ars_batch *b, ars_job *jobs, uint32_t count, uint32_t flags, v
     if (!b->jobs.empty()) std::memcpy(h + hdr.off_jobs, b->jobs.data(), b->jobs.size() * sizeof(ars::BJob));
     if (!b->preps.empty()) std::memcpy(h + hdr.off_preps, b->preps.data(), b->preps.size() * sizeof(ars::BPrep));
     if (!b->pos.empty()) std::memcpy(h + hdr.off_pos, b->pos.data(), b->pos.size());
+    // a previous run on another stream may still read the arena: order the
+    // upload after it on the device
+    if ((e = hipStreamWaitEvent(s, b->run_done, 0)) != hipSuccess) return ars::hip_error(e, "hipStreamWaitEvent");
     if ((e = hipMemcpyAsync(b->d_arena, h, stage_bytes, hipMemcpyHostToDevice, s)) != hipSuccess)
         return ars::hip_error(e, "batch descriptor upload");
     if ((e = hipEventRecord(b->upload_done, s)) != hipSuccess) return ars::hip_error(e, "hipEventRecord");
@@ -332,6 +342,7 @@ int ars_batch_run(ars_batch *b, ars_job *jobs, uint32_t count, uint32_t flags, v
             (e = ars::launch_batch(b->d_arena, b->ctx->d_tw, b->grid, s)) != hipSuccess)
             return ars::hip_error(e, "batch launch");
     }
+    if ((e = hipEventRecord(b->run_done, s)) != hipSuccess) return ars::hip_error(e, "hipEventRecord");
 
     // 6. jobs the batched kernel cannot take run as plain calls
     for (uint32_t i : fallback) {

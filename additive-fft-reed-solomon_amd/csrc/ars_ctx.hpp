@@ -26,4 +26,5 @@ bool is_pow2(uint32_t v);
 int ilog2(uint32_t v);
 void release_host_pipe(ars_ctx *ctx);     // ars_host.cpp
 void release_decode_state(ars_ctx *ctx);  // ars_decode_api.cpp
+ars_ctx *batch_context(const ars_batch *b);  // ars_batch_host.cpp
 }  // namespace ars

@@ -126,41 +126,58 @@ double secs(const ars_server *s, Clock::time_point t) {
 }
 
 void launch_slot(ars_server *s, Slot &sl) {
-    sl.jobs.assign(sl.reqs.size(), ars_job{});
+    // a request whose input copy cannot be queued fails with that error and
+    // stays out of the batch
+    std::vector<size_t> in_batch;
+    in_batch.reserve(sl.reqs.size());
+    sl.jobs.clear();
     for (size_t i = 0; i < sl.reqs.size(); ++i) {
         Req *q = sl.reqs[i];
         const ars_request &r = q->r;
         const size_t L = (size_t)r.shard_len, T = r.n - r.k;
         uint8_t *d = s->d_pool + q->off;
-        ars_job &j = sl.jobs[i];
+        ars_job j{};
         j.op = r.op;
         j.n = r.n;
         j.k = r.k;
         j.flags = r.flags;
         j.shard_len = r.shard_len;
+        hipError_t e = hipSuccess;
         if (r.op == ARS_OP_ENCODE) {
             j.d_parity = d;
             j.d_message = d + T * L;
-            if (r.host_in) hipMemcpyAsync(d + T * L, r.host_in, r.k * L, hipMemcpyHostToDevice, sl.stream);
+            if (r.host_in) e = hipMemcpyAsync(d + T * L, r.host_in, r.k * L, hipMemcpyHostToDevice, sl.stream);
         } else {
             j.d_codeword = d;
             j.positions = q->pos.data();
             j.e = r.e;
-            if (r.host_in) hipMemcpyAsync(d, r.host_in, r.n * L, hipMemcpyHostToDevice, sl.stream);
+            if (r.host_in) e = hipMemcpyAsync(d, r.host_in, r.n * L, hipMemcpyHostToDevice, sl.stream);
         }
+        if (e != hipSuccess) {
+            q->status = ars::hip_error(e, "request input copy");
+            continue;
+        }
+        in_batch.push_back(i);
+        sl.jobs.push_back(j);
     }
-    int rc = ars_batch_run(sl.batch, sl.jobs.data(), (uint32_t)sl.jobs.size(), 0, sl.stream);
-    for (size_t i = 0; i < sl.reqs.size(); ++i) {
-        Req *q = sl.reqs[i];
+    const int rc = sl.jobs.empty()
+                       ? ARS_OK
+                       : ars_batch_run(sl.batch, sl.jobs.data(), (uint32_t)sl.jobs.size(), 0, sl.stream);
+    for (size_t b = 0; b < in_batch.size(); ++b) {
+        Req *q = sl.reqs[in_batch[b]];
         const ars_request &r = q->r;
-        q->status = rc < 0 ? rc : sl.jobs[i].status;
+        q->status = rc < 0 ? rc : sl.jobs[b].status;
         if (!r.host_out || q->status < 0) continue;
         const size_t L = (size_t)r.shard_len;
         uint8_t *d = s->d_pool + q->off;
-        if (r.op == ARS_OP_ENCODE) hipMemcpyAsync(r.host_out, d, (r.n - r.k) * L, hipMemcpyDeviceToHost, sl.stream);
-        else if (q->status == ARS_TRUE) hipMemcpyAsync(r.host_out, d, r.n * L, hipMemcpyDeviceToHost, sl.stream);
+        hipError_t e = hipSuccess;
+        if (r.op == ARS_OP_ENCODE) e = hipMemcpyAsync(r.host_out, d, (r.n - r.k) * L, hipMemcpyDeviceToHost, sl.stream);
+        else if (q->status == ARS_TRUE) e = hipMemcpyAsync(r.host_out, d, r.n * L, hipMemcpyDeviceToHost, sl.stream);
+        if (e != hipSuccess) q->status = ars::hip_error(e, "request output copy");
     }
-    hipEventRecord(sl.ev, sl.stream);
+    // the slot retires (and its pool space is reused) once the event has
+    // completed; without a recorded event, wait here instead
+    if (hipEventRecord(sl.ev, sl.stream) != hipSuccess) (void)hipStreamSynchronize(sl.stream);
     sl.busy = true;
 }
 
@@ -338,12 +355,20 @@ int ars_server_wait(ars_server *s, uint64_t ticket, int32_t *status, double *lat
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->tickets.find(ticket);
     if (it == s->tickets.end()) return ars::set_error(ARS_E_STATE, "unknown or already collected ticket");
+    // another thread may collect (and free) the same ticket meanwhile: look
+    // it up again after every wake-up
+    s->cv_done.wait(lk, [s, ticket] {
+        auto f = s->tickets.find(ticket);
+        return f == s->tickets.end() || f->second->done;
+    });
+    it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return ars::set_error(ARS_E_STATE, "ticket collected by another thread");
     Req *q = it->second;
-    s->cv_done.wait(lk, [q] { return q->done; });
     if (status) *status = q->status;
     if (latency_s) *latency_s = std::chrono::duration<double>(q->t_done - q->t_submit).count();
     s->tickets.erase(it);
     delete q;
+    s->cv_done.notify_all();  // a second waiter on this ticket returns
     return ARS_OK;
 }
 

@@ -63,14 +63,15 @@ int ars_autotune(ars_ctx *ctx, uint32_t n, uint32_t k, size_t L, ars_tune_result
         if (a) hipEventDestroy(a);
         return ars::hip_error(e, "autotune scratch");
     }
-    hipMemsetAsync(buf, 0x5a, (size_t)n * L, s);
+    e = hipMemsetAsync(buf, 0x5a, (size_t)n * L, s);
     // a plan for e = T erasures at every other position from 1 (the
     // reference's test pattern, rs/src/poly_11d_lut.rs:546-555)
     ars::PosArg pa;
     uint32_t ecount = 0;
     for (uint32_t p = 1; p < n && ecount < T; p += 2) pa.p[ecount++] = (uint8_t)p;
     for (uint32_t p = 0; p < n && ecount < T; p += 2) pa.p[ecount++] = (uint8_t)p;
-    e = ars::launch_recover_prep(reinterpret_cast<ars::RecPrep *>(plan), n, ecount, pa, ctx->d_gtab, s);
+    if (e == hipSuccess)
+        e = ars::launch_recover_prep(reinterpret_cast<ars::RecPrep *>(plan), n, ecount, pa, ctx->d_gtab, s);
 
     for (int op = 0; op < 2 && e == hipSuccess; ++op) {
         ars_tune_result res{};
@@ -127,7 +128,8 @@ int ars_batch_autotune(ars_batch *bt, ars_job *jobs, uint32_t count, uint32_t *b
     if (!bt || !jobs) return ars::set_error(ARS_E_NULL, "null batch or jobs");
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipEvent_t a = nullptr, b = nullptr;
-    hipError_t e;
+    hipError_t e = hipSetDevice(ars::batch_context(bt)->device);
+    if (e != hipSuccess) return ars::hip_error(e, "hipSetDevice");
     if ((e = hipEventCreate(&a)) != hipSuccess || (e = hipEventCreate(&b)) != hipSuccess) {
         if (a) hipEventDestroy(a);
         return ars::hip_error(e, "hipEventCreate");
@@ -138,12 +140,12 @@ int ars_batch_autotune(ars_batch *bt, ars_job *jobs, uint32_t count, uint32_t *b
         ars_batch_set_blocks_per_cu(bt, bpc);
         int rc = ars_batch_run(bt, jobs, count, 0, stream);  // warm-up and graph capture
         if (rc) break;
-        hipEventRecord(a, s);
-        for (int i = 0; i < 3; ++i) ars_batch_run(bt, jobs, count, 0, stream);
-        hipEventRecord(b, s);
-        hipEventSynchronize(b);
+        if (hipEventRecord(a, s) != hipSuccess) break;
+        for (int i = 0; i < 3 && rc == ARS_OK; ++i) rc = ars_batch_run(bt, jobs, count, 0, stream);
         float ms = 0;
-        hipEventElapsedTime(&ms, a, b);
+        if (rc || hipEventRecord(b, s) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+            hipEventElapsedTime(&ms, a, b) != hipSuccess)
+            break;
         if (best < 0 || ms < best) {
             best = ms;
             best_b = bpc;

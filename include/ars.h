@@ -166,8 +166,10 @@ typedef struct ars_batch_info {
 #define ARS_BATCH_NO_GRAPH 1u  /* launch the kernels directly instead of replaying the graph */
 
 /* A batch object owns a device arena for descriptors and erasure plans and a
- * captured graph; it serves one run at a time (runs on one object are
- * ordered on their streams; use several objects for overlap). */
+ * captured graph; it serves one run at a time: a run's descriptor upload
+ * waits, on the device, for the previous run on the same object, whatever
+ * stream that used (use several objects for overlap).  Calls on one object
+ * from several host threads at once are not supported. */
 int ars_batch_create(ars_ctx *ctx, uint32_t max_jobs, ars_batch **out);
 int ars_batch_destroy(ars_batch *b);
 /* Validates every job (setting its status), then launches all valid jobs as
@@ -265,6 +267,9 @@ typedef struct ars_tune_result {
 int ars_autotune(ars_ctx *ctx, uint32_t n, uint32_t k, size_t shard_len, ars_tune_result *encode_out,
                  ars_tune_result *recover_out, void *stream);
 int ars_tune_set(ars_ctx *ctx, uint32_t op, uint32_t log2_size, uint32_t quads, uint32_t buffers);
+/* Runs the batch 16 times on the jobs' own buffers (encode rewrites the
+ * parity, recover the erased shards, with the same result every time) and
+ * keeps the fastest blocks-per-CU setting in the batch object. */
 int ars_batch_autotune(ars_batch *b, ars_job *jobs, uint32_t count, uint32_t *best_blocks_per_cu,
                        void *stream);
 

@@ -143,3 +143,24 @@ def test_batch_statuses_and_fallback(ars):
     assert b.info()["fallback_jobs"] == 1
     assert np.array_equal(d.cpu().numpy(), cw)
     assert np.array_equal(good.cpu().numpy(), cw)
+
+
+def test_batch_object_runs_on_two_streams(ars):
+    """Runs of one batch object issued back-to-back on two streams, with no
+    host synchronisation between them: each run's descriptor upload waits on
+    the device for the previous run, so every run reads its own jobs."""
+    b = ars.Batch(128)
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    runs = []
+    for rep, count in enumerate((48, 12, 40, 8)):
+        rng = np.random.default_rng(700 + rep)
+        lens = [65536, 16384] if rep == 0 else [64, 256, 1024]
+        specs, checks = make_mixed(rng, count, [(256, 128), (64, 32), (128, 96), (32, 16)], lens)
+        torch.cuda.synchronize()  # inputs are on the device before the runs start
+        st = b.run(specs, stream=streams[rep % 2])
+        runs.append((specs, checks, st))
+    torch.cuda.synchronize()
+    for rep, (specs, checks, st) in enumerate(runs):
+        for i, ((buf, exp, want), s) in enumerate(zip(checks, st)):
+            assert s == want, (rep, i)
+            assert np.array_equal(buf.cpu().numpy(), exp), (rep, i, specs[i]["op"], specs[i]["n"])

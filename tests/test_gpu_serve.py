@@ -120,3 +120,32 @@ def test_server_replay_statuses(ars):
     assert list(st) == [0, 0, ars.ARS_E_DUPLICATE, 1, 0, 1]
     assert (lat > 0).all()
     srv.close()
+
+
+def test_server_two_waiters_one_ticket(ars):
+    """Two threads waiting on one ticket: one collects the status, the other
+    is told the ticket is gone (never a freed request)."""
+    srv = ars.Server(hbm_budget=256 << 20, max_batch_jobs=8)
+    rng = np.random.default_rng(47)
+    for rep in range(4):
+        inp, out, exp, _ = make_request_data(rng, 256, 128, 1 << 16, "encode")
+        keep = [inp, out]
+        t = srv.submit(ars.Server.request("encode", 256, 128, 1 << 16, host_in=inp, host_out=out, keep=keep))
+        got = []
+
+        def waiter():
+            try:
+                got.append(("ok",) + tuple(srv.wait(t)))
+            except ars.ArsError as err:
+                got.append(("err", err.code))
+
+        th = [threading.Thread(target=waiter) for _ in range(2)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join(timeout=60)
+        assert sorted(g[0] for g in got) == ["err", "ok"], got
+        assert [g[1] for g in got if g[0] == "err"] == [ars.ARS_E_STATE]
+        assert [g[1] for g in got if g[0] == "ok"] == [0]
+        assert np.array_equal(out, exp)
+    srv.close()
