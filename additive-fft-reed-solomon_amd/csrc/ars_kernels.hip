@@ -47,7 +47,9 @@ __global__ void __launch_bounds__(256) k_encode_small(const uint8_t *__restrict_
     body_encode_small<TL, ACC>(io, chunks, tw);
 }
 
-template <int TL, int PQ, bool AL, bool ACC, int NBUF>
+// ONE: the message is a single chunk (k == T), known at compile time, so the
+// chunk loop and its accumulator disappear (fewer VGPRs, higher occupancy).
+template <int TL, int PQ, bool AL, bool ACC, int NBUF, bool ONE>
 __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
     k_encode_tile(const uint8_t *__restrict__ msg, uint8_t *__restrict__ par, uint32_t L,
                   uint32_t chunks, const MulTab *__restrict__ g_tw) {
@@ -58,7 +60,7 @@ __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
     __syncthreads();
     const int p = threadIdx.x % PQ, q = tile_q<PQ>();
     BufIO<AL> io = make_bufio<AL>(msg, chunks * TT::N * L, par, TT::N * L, blockIdx.x * PQ + p, L);
-    body_encode_tile<TL, PQ, ACC, NBUF>(io, q, p, tile, chunks, tw);
+    body_encode_tile<TL, PQ, ACC, NBUF>(io, q, p, tile, ONE ? 1u : chunks, tw);
 }
 
 template <int KL, bool AL>
@@ -179,8 +181,12 @@ static hipError_t enc_one(const uint8_t *msg, uint8_t *par, uint32_t L, uint32_t
     } else {
 #define ENC_V(PQ_, NB_)                                                                              \
     if (pq == PQ_ && nbuf == NB_) {                                                                  \
-        hipLaunchKernelGGL((k_encode_tile<TL, PQ_, AL, ACC, NB_>), dim3(ceil_div(quads, PQ_)),       \
-                           dim3(Tile<TL, PQ_>::THREADS), 0, s, msg, par, L, chunks, tw);             \
+        if (chunks == 1)                                                                             \
+            hipLaunchKernelGGL((k_encode_tile<TL, PQ_, AL, ACC, NB_, true>), dim3(ceil_div(quads, PQ_)), \
+                               dim3(Tile<TL, PQ_>::THREADS), 0, s, msg, par, L, chunks, tw);         \
+        else                                                                                         \
+            hipLaunchKernelGGL((k_encode_tile<TL, PQ_, AL, ACC, NB_, false>), dim3(ceil_div(quads, PQ_)), \
+                               dim3(Tile<TL, PQ_>::THREADS), 0, s, msg, par, L, chunks, tw);         \
         return hipGetLastError();                                                                    \
     }
         ENC_V(16, 1) ENC_V(16, 2) ENC_V(32, 1) ENC_V(32, 2) ENC_V(64, 1) ENC_V(64, 2)

@@ -515,7 +515,7 @@ def main():
     enc_gbps = n * L / s_enc / 1e9
     rec_gbps = (n + e) * L / s_rec / 1e9
     # the tile variants the autotuner kept (quads per workgroup, LDS buffers)
-    enc_name = f"k_encode_tile<7, {tune_enc['quads']}, true, false, {tune_enc['buffers']}>"
+    enc_name = f"k_encode_tile<7, {tune_enc['quads']}, true, false, {tune_enc['buffers']}, true>"
     rec_name = f"k_recover_tile<8, {tune_rec['quads']}, true, {tune_rec['buffers']}>"
     if s_rec >= s_enc:
         dom, dom_bytes, dom_s = rec_name, (n + e) * L, s_rec
