@@ -141,8 +141,10 @@ def measured_traffic(kernel):
 
 def probe_valu_rates():
     """Live VALU issue rates (wave64 instructions / s) from libars_probe.so:
-    independent v_xor_b32 chains, v_perm_b32 chains, and the fused kernels'
-    butterfly loop on registers.  None if the probe library is missing."""
+    independent v_xor_b32 chains, v_perm_b32 chains, the fused kernels'
+    butterfly in a small loop on registers, and the same butterflies in
+    straight-line code the size of a tile body.  None if the probe library
+    is missing."""
     import ctypes
     path = os.path.join(ROOT, "additive-fft-reed-solomon_amd", "libars_probe.so")
     if not os.path.exists(path):
@@ -150,10 +152,10 @@ def probe_valu_rates():
     lib = ctypes.CDLL(path)
     lib.ars_probe_valu_rates.restype = ctypes.c_int
     lib.ars_probe_valu_rates.argtypes = [ctypes.POINTER(ctypes.c_double)]
-    out = (ctypes.c_double * 3)()
+    out = (ctypes.c_double * 4)()
     if lib.ars_probe_valu_rates(out) != 0:
         return None
-    return {"simple": out[0], "perm": out[1], "butterfly_loop": out[2]}
+    return {"simple": out[0], "perm": out[1], "butterfly_loop": out[2], "butterfly_straight_line": out[3]}
 
 
 def valu_roofline(kernel, seconds, rates):
@@ -178,6 +180,9 @@ def valu_roofline(kernel, seconds, rates):
     return {"bound": "valu", "kernel": name, "achieved": round(achieved / 1e12, 3), "peak": round(peak / 1e12, 3),
             "unit": "T wave64-instr/s", "frac": round(achieved / peak, 4), "perm_fraction": fam,
             "butterfly_loop_frac": round(rates["butterfly_loop"] / peak, 4),
+            # the kernel against the same multiply-add butterflies issued in
+            # straight-line code (its instruction mix is that butterfly's)
+            "frac_of_straight_line_butterfly_rate": round(achieved / rates["butterfly_straight_line"], 4),
             "peak_from": "live v_xor_b32 and v_perm_b32 issue rates weighted by the kernel's v_perm share"}
 
 
