@@ -4,8 +4,14 @@
 // item is one tile of one code shape whose lanes carry column quads of one or
 // more codewords (BSeg), so a large codeword spans many items and many small
 // codewords share one.  A fixed grid of workgroups (sized to the occupancy)
-// pulls items from an atomic counter until the queue is empty.  Twiddles are
-// indexed by absolute point index, so one LDS copy serves every code shape.
+// walks the regions -- one per transform size, body variant and addressing,
+// largest transforms first -- pulling each region's items from its own
+// atomic counter until that region is drained.  Each region is its own
+// inlined loop around one tile body, so no body's register allocation
+// constrains another's and no body sits behind a function call (a call
+// would pass the buffer descriptors in VGPRs and save registers to scratch).
+// Twiddles are indexed by absolute point index, so one LDS copy serves every
+// code shape.
 //
 // The three launches of a batch -- counter reset, erasure plans, persistent
 // kernel -- read their sizes from a device header, so a hipGraph captured
@@ -36,7 +42,8 @@ struct PackIO {
 };
 
 __global__ void k_batch_reset(uint8_t *arena) {
-    if (threadIdx.x == 0) reinterpret_cast<BHeader *>(arena)->counter = 0;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kBatchRegions; i += blockDim.x)
+        reinterpret_cast<BHeader *>(arena)->counters[i] = 0;
 }
 
 // Erasure plans for every recover job of the batch (fixed grid, loops).
@@ -95,53 +102,81 @@ __device__ __forceinline__ void stage_item_plans(const BItem &it, const BSeg *__
     }
 }
 
-template <int TL, class IO>
-__device__ __noinline__ void run_body(const BItem &it, IO &io, int q, int p, uint32_t *tile,
+// Descriptor fields are wave-uniform, but values read with vector loads from
+// the (atomically updated) arena sit in VGPRs; a buffer resource built from
+// them would force a readfirstlane loop around every load and store.  These
+// helpers move them into SGPRs.
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+template <class T>
+__device__ __forceinline__ T *uni(T *ptr) {
+    const uint64_t v = reinterpret_cast<uint64_t>(ptr);
+    const uint64_t lo = uni((uint32_t)v), hi = uni((uint32_t)(v >> 32));
+    return reinterpret_cast<T *>(lo | (hi << 32));
+}
+__device__ __forceinline__ BItem uni(const BItem &x) {
+    BItem y;
+    y.meta = uni(x.meta);
+    y.arg = uni(x.arg);
+    y.seg0 = uni(x.seg0);
+    y.nseg_rep = uni(x.nseg_rep);
+    y.in_base = uni(x.in_base);
+    y.out_base = uni(x.out_base);
+    return y;
+}
+__device__ __forceinline__ BJob uni(const BJob &x) {
+    BJob y;
+    y.in = uni(x.in);
+    y.out = uni(x.out);
+    y.plan = uni(x.plan);
+    y.L = uni(x.L);
+    y.quads = uni(x.quads);
+    y.in_bytes = uni(x.in_bytes);
+    y.out_bytes = uni(x.out_bytes);
+    return y;
+}
+
+// Tile body of variant V (see batch_region) for transform size 2^TL.
+template <int TL, int V, class IO>
+__device__ __forceinline__ void run_body(const BItem &it, IO &io, int q, int p, uint32_t *tile,
                                          const MulTab *tw, const MulTab *s_tab, const uint8_t *s_er,
                                          uint32_t si) {
-    const uint32_t kind = it.meta & 0xffu;
-    const bool acc = (it.meta >> 16) & 1u;
     constexpr uint32_t PQ = batch_lanes(TL);
     if constexpr (TL <= 4) {
-        if (kind == kEncRef) {
-            if (acc) body_encode_small<TL, true>(io, it.arg, tw);
-            else body_encode_small<TL, false>(io, it.arg, tw);
-        } else if (kind == kEncKpow2) {
+        if constexpr (V == 0 || V == 1) {
+            body_encode_small<TL, V == 1>(io, it.arg, tw);
+        } else if constexpr (V == 2) {
             body_encode_kpow2_small<TL>(io, it.arg & 0xffffu, it.arg >> 16, tw);
         } else if constexpr (TL >= 1) {
             LdsTabs tabs{s_tab + si * (1u << TL), s_er + si * (1u << TL)};
             body_recover_small<TL>(io, tabs, tw);
         }
+    } else if constexpr (V == 0 || V == 1) {
+        if constexpr (TL <= 7) body_encode_tile<TL, PQ, V == 1, 1>(io, q, p, tile, it.arg, tw);
+    } else if constexpr (V == 2) {
+        if constexpr (TL <= 7) body_encode_kpow2_tile<TL, PQ, 1>(io, q, p, tile, it.arg & 0xffffu, it.arg >> 16, tw);
     } else {
-        if (kind == kEncRef) {
-            if constexpr (TL <= 7) {
-                if (acc) body_encode_tile<TL, PQ, true, 1>(io, q, p, tile, it.arg, tw);
-                else body_encode_tile<TL, PQ, false, 1>(io, q, p, tile, it.arg, tw);
-            }
-        } else if (kind == kEncKpow2) {
-            if constexpr (TL <= 7) body_encode_kpow2_tile<TL, PQ, 1>(io, q, p, tile, it.arg & 0xffffu, it.arg >> 16, tw);
-        } else {
-            LdsTabs tabs{s_tab + si * (1u << TL), s_er + si * (1u << TL)};
-            body_recover_tile<TL, PQ, 1>(io, tabs, q, p, tile, tw);
-        }
+        LdsTabs tabs{s_tab + si * (1u << TL), s_er + si * (1u << TL)};
+        body_recover_tile<TL, PQ, 1>(io, tabs, q, p, tile, tw);
     }
 }
 
-template <int TL>
-__device__ __forceinline__ void run_tile(const BItem &it, const BSeg *segs, const BJob *jobs,
-                                         uint32_t r, uint32_t *tile, const MulTab *tw,
-                                         const MulTab *s_tab, const uint8_t *s_er) {
+// Repetition r of an item: one codeword (PACK = 0: scalar job descriptor,
+// buffer-resource addressing) or several (PACK = 1: per-lane offsets from the
+// item's base pointers).
+template <int TL, int V, int PACK>
+__device__ __forceinline__ void run_tile(const BItem &it, const BSeg *segs, const BJob *jobs, uint32_t r,
+                                         uint32_t *tile, const MulTab *tw, const MulTab *s_tab,
+                                         const uint8_t *s_er) {
     constexpr uint32_t PQ = batch_lanes(TL);
     const int p = (int)(threadIdx.x % PQ);
     const int q = (int)(threadIdx.x / PQ);
-    if ((it.nseg_rep & 0xffffu) == 1u) {
-        // one codeword: scalar job descriptor, buffer-resource addressing
-        const BSeg sg = segs[it.seg0];
-        const BJob job = jobs[sg.job];
-        const uint32_t quad = sg.quad0 + (uint32_t)p + r * PQ;
+    if constexpr (PACK == 0) {
+        const uint32_t jid = uni(segs[it.seg0].job), quad0 = uni(segs[it.seg0].quad0);
+        const BJob job = uni(jobs[jid]);
+        const uint32_t quad = quad0 + (uint32_t)p + r * PQ;
         BufIO<true> io = make_bufio<true>(job.in, job.in_bytes, job.out, job.out_bytes,
                                           quad < job.quads ? quad : job.quads, job.L);
-        run_body<TL>(it, io, q, p, tile, tw, s_tab, s_er, 0u);
+        run_body<TL, V>(it, io, q, p, tile, tw, s_tab, s_er, 0u);
     } else if constexpr (TL <= kBatchPackMaxTl) {
         BJob job;
         uint32_t col, nb, si;
@@ -153,26 +188,63 @@ __device__ __forceinline__ void run_tile(const BItem &it, const BSeg *segs, cons
         io.vout = (uint32_t)(job.out - it.out_base) + col;
         io.L = job.L;
         io.nb = nb;
-        run_body<TL>(it, io, q, p, tile, tw, s_tab, s_er, si);
+        run_body<TL, V>(it, io, q, p, tile, tw, s_tab, s_er, si);
     }
 }
 
-template <int TL>
-__device__ __forceinline__ void run_item(const BItem &it, const BSeg *segs, const BJob *jobs,
-                                         uint32_t *tile, const MulTab *tw, MulTab *s_tab, uint8_t *s_er) {
-    const uint32_t rep = it.nseg_rep >> 16;
-    if ((it.meta & 0xffu) == kRecover) {
-        stage_item_plans<TL>(it, segs, jobs, s_tab, s_er);
+// Drain region (TL, V, PACK): pull its items until its counter passes its
+// range.  The two barriers around the item index also order the previous
+// item's LDS reads (tile, plans) before this item's writes.
+template <int TL, int V, int PACK>
+__device__ __forceinline__ void run_region(BHeader *h, const BItem *items, const BSeg *segs, const BJob *jobs,
+                                           uint32_t *tile, const MulTab *tw, MulTab *s_tab, uint8_t *s_er,
+                                           uint32_t *s_item) {
+    constexpr uint32_t g = batch_region(TL, V, PACK);
+    const uint32_t begin = uni(h->begin[g]), end = uni(h->end[g]);
+    if (begin >= end) return;
+    for (;;) {
+        if (threadIdx.x == 0) *s_item = begin + atomicAdd(&h->counters[g], 1u);
         __syncthreads();
-    }
-    for (uint32_t r = 0; r < rep; ++r) {
-        if (r) __syncthreads();  // the previous tile's LDS reads are done
-        run_tile<TL>(it, segs, jobs, r, tile, tw, s_tab, s_er);
+        const uint32_t i = uni(*s_item);
+        __syncthreads();
+        if (i >= end) break;
+        const BItem it = uni(items[i]);
+        if constexpr (V == 3) {
+            stage_item_plans<TL>(it, segs, jobs, s_tab, s_er);
+            __syncthreads();
+        }
+        const uint32_t rep = it.nseg_rep >> 16;
+        for (uint32_t r = 0; r < rep; ++r) {
+            if (r) __syncthreads();  // the previous tile's LDS reads are done
+            run_tile<TL, V, PACK>(it, segs, jobs, r, tile, tw, s_tab, s_er);
+        }
     }
 }
 
-__global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) k_batch(uint8_t *__restrict__ arena,
-                                                         const MulTab *__restrict__ g_tw) {
+// Every region of transform size 2^TL, then the next smaller size.
+template <int TL>
+__device__ __forceinline__ void run_sizes(BHeader *h, const BItem *items, const BSeg *segs, const BJob *jobs,
+                                          uint32_t *tile, const MulTab *tw, MulTab *s_tab, uint8_t *s_er,
+                                          uint32_t *s_item) {
+    if constexpr (TL >= 0) {
+        if constexpr (TL <= 7) {  // encode transforms are at most 128 points
+            run_region<TL, 0, 0>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
+            run_region<TL, 1, 0>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
+            run_region<TL, 2, 0>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
+        }
+        if constexpr (TL >= 1) run_region<TL, 3, 0>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
+        if constexpr (TL <= kBatchPackMaxTl) {
+            run_region<TL, 0, 1>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
+            run_region<TL, 1, 1>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
+            run_region<TL, 2, 1>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
+            if constexpr (TL >= 1) run_region<TL, 3, 1>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
+        }
+        run_sizes<TL - 1>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
+    }
+}
+
+__global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4, 8)))
+    k_batch(uint8_t *__restrict__ arena, const MulTab *__restrict__ g_tw) {
     __shared__ uint32_t tile[8192];  // 2^TL shards x batch_lanes(TL) quads, every TL >= 5
     __shared__ MulTab tw[kTwiddleEntries];
     __shared__ MulTab s_tab[kBatchPlanSlots];
@@ -187,27 +259,8 @@ __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per
         uint32_t *dst = reinterpret_cast<uint32_t *>(tw);
         for (uint32_t i = threadIdx.x; i < (uint32_t)kTwiddleEntries * 6u; i += blockDim.x) dst[i] = src[i];
     }
-    const uint32_t nitems = h->nitems;
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(&h->counter, 1u);
-        __syncthreads();
-        const uint32_t i = __builtin_amdgcn_readfirstlane(s_item);
-        __syncthreads();
-        if (i >= nitems) break;
-        const BItem it = items[i];
-        switch ((it.meta >> 8) & 0xffu) {
-            case 0: run_item<0>(it, segs, jobs, tile, tw, s_tab, s_er); break;
-            case 1: run_item<1>(it, segs, jobs, tile, tw, s_tab, s_er); break;
-            case 2: run_item<2>(it, segs, jobs, tile, tw, s_tab, s_er); break;
-            case 3: run_item<3>(it, segs, jobs, tile, tw, s_tab, s_er); break;
-            case 4: run_item<4>(it, segs, jobs, tile, tw, s_tab, s_er); break;
-            case 5: run_item<5>(it, segs, jobs, tile, tw, s_tab, s_er); break;
-            case 6: run_item<6>(it, segs, jobs, tile, tw, s_tab, s_er); break;
-            case 7: run_item<7>(it, segs, jobs, tile, tw, s_tab, s_er); break;
-            case 8: run_item<8>(it, segs, jobs, tile, tw, s_tab, s_er); break;
-            default: break;
-        }
-    }
+    __syncthreads();
+    run_sizes<8>(h, items, segs, jobs, tile, tw, s_tab, s_er, &s_item);
 }
 
 hipError_t launch_batch_reset(uint8_t *arena, hipStream_t s) {

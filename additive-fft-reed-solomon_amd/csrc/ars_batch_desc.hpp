@@ -65,12 +65,26 @@ struct alignas(16) BPrep {
     uint32_t n, e, pos_off, pad;
 };
 
+// Regions of the persistent kernel: one per (transform size tl, body
+// variant, addressing), so each tile body runs in a loop of its own.
+// Variant: 0 encode, 1 encode with accumulate, 2 encode k = 2^KL, 3 recover.
+// Addressing: 0 one codeword per item (scalar descriptor), 1 packed item.
+constexpr int kBatchClasses = 9;
+constexpr int kBatchVariants = 4;
+constexpr int kBatchRegions = kBatchClasses * kBatchVariants * 2;
+constexpr uint32_t batch_region(uint32_t tl, uint32_t variant, uint32_t packed) {
+    return (tl * kBatchVariants + variant) * 2u + packed;
+}
+
 // Header at the start of the device arena (read by the graph's kernels, so
 // the captured launches never change while the work behind them does).
-// Offsets are in bytes from the arena base.
+// Offsets are in bytes from the arena base.  Items are sorted by region in
+// the order the kernel walks them (largest transforms first); region g owns
+// items [begin[g], end[g]) and hands them out through counters[g].
 struct alignas(16) BHeader {
-    uint32_t nitems, nprep, counter, pad;
+    uint32_t nitems, nprep, pad0, pad1;
     uint64_t off_items, off_segs, off_jobs, off_preps, off_pos, off_plans;
+    uint32_t begin[kBatchRegions], end[kBatchRegions], counters[kBatchRegions];
 };
 
 // Lanes per item: tile kinds use kBatchThreads / Q column quads.

@@ -291,6 +291,7 @@ int ars_batch_run(ars_batch *b, ars_job *jobs, uint32_t count, uint32_t flags, v
     ars::BHeader hdr{};
     hdr.nitems = (uint32_t)b->items.size();
     hdr.nprep = (uint32_t)b->preps.size();
+    ars::order_regions(b->items, hdr.begin, hdr.end);
     size_t off = align_up(sizeof(ars::BHeader));
     hdr.off_items = off;
     off = align_up(off + b->items.size() * sizeof(ars::BItem));

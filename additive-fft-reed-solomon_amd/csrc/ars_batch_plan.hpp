@@ -88,6 +88,35 @@ inline void plan_items(std::vector<Planned> &planned, const std::vector<BJob> &j
     }
 }
 
+// Region of an item (batch_region): its transform size, body variant and
+// addressing.  The kernel walks regions largest transform first, then by
+// variant, then single-codeword before packed items.
+inline uint32_t item_region(const BItem &it) {
+    const uint32_t kind = it.meta & 0xffu, tl = (it.meta >> 8) & 0xffu, acc = (it.meta >> 16) & 1u;
+    const uint32_t variant = kind == kEncRef ? acc : kind == kEncKpow2 ? 2u : 3u;
+    return batch_region(tl, variant, (it.nseg_rep & 0xffffu) > 1u ? 1u : 0u);
+}
+inline uint32_t region_rank(uint32_t g) {  // position of region g in the kernel's walk
+    const uint32_t tl = g / (2u * kBatchVariants), rest = g % (2u * kBatchVariants);
+    return (uint32_t)(kBatchClasses - 1 - tl) * (2u * kBatchVariants) + rest;
+}
+
+// Reorders items into the kernel's region order (stable: within a region the
+// planner's order stays), then fills each region's item range; an empty
+// region gets begin == end.  Items keep their segment indices, so the
+// segment array is unchanged.
+inline void order_regions(std::vector<BItem> &items, uint32_t *begin, uint32_t *end) {
+    std::stable_sort(items.begin(), items.end(), [](const BItem &a, const BItem &b) {
+        return region_rank(item_region(a)) < region_rank(item_region(b));
+    });
+    for (int g = 0; g < kBatchRegions; ++g) begin[g] = end[g] = 0;
+    for (size_t i = 0; i < items.size(); ++i) {
+        const uint32_t g = item_region(items[i]);
+        if (begin[g] == end[g]) begin[g] = (uint32_t)i;
+        end[g] = (uint32_t)i + 1;
+    }
+}
+
 // Walks the items exactly as the persistent kernel maps lanes to codeword
 // quads and checks that every quad of every planned job is covered exactly
 // once, that every item stays within its shape's lanes and segment limit,

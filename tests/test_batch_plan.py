@@ -1,9 +1,11 @@
 """CPU unit test of the batch work-item planner (csrc/ars_batch_plan.hpp):
 for random mixes of codeword shapes and shard lengths, every column quad of
 every job is covered exactly once by the lane mapping the persistent kernel
-uses, items respect their shape's lane and segment limits, and packed
-codewords fit the 32-bit offset window.  Also checks that the verifier
-catches planted mistakes."""
+uses, items respect their shape's lane and segment limits, packed codewords
+fit the 32-bit offset window, and the region order the kernel walks (one
+contiguous item range per transform size, body variant and addressing)
+covers every item.  Also checks that the verifier catches planted
+mistakes."""
 import os
 import shutil
 import subprocess
@@ -54,6 +56,25 @@ static int run(unsigned seed, int count, bool plant) {
     std::vector<BItem> items;
     std::vector<BSeg> segs;
     plan_items(planned, jobs, items, segs);
+    // region order the persistent kernel walks: every item in its region's
+    // range, ranges in the kernel's walk order, nothing left out
+    uint32_t rb[kBatchRegions], re[kBatchRegions];
+    order_regions(items, rb, re);
+    size_t covered = 0;
+    uint32_t last_rank = 0, last_end = 0;
+    for (uint32_t rank = 0; rank < (uint32_t)kBatchRegions; ++rank) {
+        for (uint32_t g = 0; g < (uint32_t)kBatchRegions; ++g) {
+            if (region_rank(g) != rank || rb[g] == re[g]) continue;
+            if (rb[g] != last_end) { std::printf("seed %u: region %u starts at %u, expected %u\n", seed, g, rb[g], last_end); return 1; }
+            for (uint32_t i = rb[g]; i < re[g]; ++i)
+                if (item_region(items[i]) != g) { std::printf("seed %u: item %u outside its region\n", seed, i); return 1; }
+            covered += re[g] - rb[g];
+            last_end = re[g];
+            last_rank = rank;
+        }
+    }
+    (void)last_rank;
+    if (covered != items.size()) { std::printf("seed %u: regions cover %zu of %zu items\n", seed, covered, items.size()); return 1; }
     if (plant && !items.empty()) {
         // planted mistake: drop the last item, or shift one segment's lanes
         if (seed % 2) items.pop_back(); else segs[items[0].seg0].quad0 += 1;
