@@ -1,0 +1,95 @@
+"""Static checks on the gfx950 assembly of the kernels (CPU only: hipcc
+cross-compiles).  They guard against code shapes that cost more than the
+arithmetic, each of which has happened in this code base:
+  * a buffer load or store wrapped in a readfirstlane / exec loop, which the
+    compiler emits when a buffer descriptor sits in VGPRs;
+  * function calls in a kernel (the descriptors cross the call in VGPRs and
+    the callee saves its callee-saved VGPRs to scratch on every call);
+  * flat loads / stores in the one-codeword tile kernels, i.e. LDS
+    addressed through generic pointers;
+  * scratch in the kernels the library launches by default."""
+import os
+import re
+import shutil
+import subprocess
+from concurrent.futures import ThreadPoolExecutor
+
+import pytest
+
+import _pkg
+
+CSRC = os.path.join(_pkg.PKG_DIR, "csrc")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
+         "-packed-fp32-ops", "-S", "--cuda-device-only"]
+SOURCES = ["ars_kernels.hip", "ars_batch.hip", "ars_decode.hip", "ars_host.cpp"]
+
+
+def functions(asm):
+    """{symbol: list of lines} for every function in a device assembly file."""
+    lines = asm.split("\n")
+    starts = [i for i, l in enumerate(lines) if re.match(r"^_Z[A-Za-z0-9_]+:", l)]
+    starts.append(len(lines))
+    return {lines[a].split(":")[0]: lines[a:b] for a, b in zip(starts, starts[1:])}
+
+
+@pytest.fixture(scope="module")
+def asm(tmp_path_factory):
+    if not os.path.exists(HIPCC) and not shutil.which("hipcc"):
+        pytest.skip("hipcc missing")
+    out = tmp_path_factory.mktemp("isa")
+
+    def one(src):
+        dst = os.path.join(out, src + ".s")
+        cmd = [HIPCC] + FLAGS + (["-x", "hip"] if src.endswith(".cpp") else []) + \
+            [os.path.join(CSRC, src), "-o", dst]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-2000:]
+        with open(dst) as f:
+            return src, functions(f.read())
+
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        return dict(ex.map(one, SOURCES))
+
+
+def waterfalls(body):
+    return sum(1 for i, l in enumerate(body[:-1])
+               if "s_and_saveexec_b64" in l and re.search(r"buffer_(load|store)", body[i + 1]))
+
+
+def test_no_readfirstlane_loops_around_buffer_access(asm):
+    bad = {f"{src}:{name[:80]}": n for src, fns in asm.items() for name, body in fns.items()
+           if (n := waterfalls(body))}
+    assert not bad, bad
+
+
+def test_no_calls_in_kernels(asm):
+    bad = [f"{src}:{name[:80]}" for src, fns in asm.items() for name, body in fns.items()
+           if any("s_swappc_b64" in l for l in body)]
+    assert not bad, bad
+
+
+def test_tile_kernels_address_lds_directly(asm):
+    # (the batched kernels read their arena descriptors, which are global,
+    # through generic pointers; flat loads there are not LDS accesses)
+    bad = [name[:80] for name, body in asm["ars_kernels.hip"].items()
+           if "tile" in name and any(re.search(r"\bflat_(load|store)", l) for l in body)]
+    assert not bad, bad
+
+
+def test_default_kernels_do_not_spill(asm):
+    # the variants the launchers pick without tuning: encode PQ 64 / NBUF 2
+    # (PQ 32 for 256 points), recover PQ 64 / NBUF 2 (PQ 32 / NBUF 1 for 256)
+    # and the autotuner's usual picks
+    want = [r"k_encode_tileILi7ELi32ELb1ELb0ELi1ELb1E", r"k_encode_tileILi7ELi64ELb1ELb0ELi2E",
+            r"k_recover_tileILi8ELi32ELb1ELi1E", r"k_recover_tileILi7ELi64ELb1ELi2E",
+            r"k_syndrome_tile", r"k_decode_columns"]
+    seen = {}
+    for src, fns in asm.items():
+        for name, body in fns.items():
+            for w in want:
+                if re.search(w, name):
+                    m = re.search(r"; ScratchSize: (\d+)", "\n".join(body))
+                    seen[name[:90]] = int(m.group(1)) if m else 0
+    assert len(seen) >= len(want), seen
+    assert all(v == 0 for v in seen.values()), seen
