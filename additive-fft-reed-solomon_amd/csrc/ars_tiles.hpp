@@ -88,9 +88,12 @@ __device__ __forceinline__ void body_encode_small(IO &io, uint32_t chunks, const
 
 // NBUF = 2: ping-pong LDS buffers (one barrier per exchange); NBUF = 1: one
 // buffer and an extra barrier before each reuse.
+// x0 (optional): chunk 0's 16 input rows, already loaded by the caller (the
+// one-codeword kernels issue those loads before staging their tables, so the
+// two latencies overlap).
 template <int TL, int PQ, bool ACC, int NBUF, class IO>
 __device__ __forceinline__ void body_encode_tile(IO &io, int q, int p, uint32_t *tile, uint32_t chunks,
-                                                 const MulTab *tw) {
+                                                 const MulTab *tw, const uint32_t *x0 = nullptr) {
     using TT = Tile<TL, PQ>;
     constexpr int T = TT::N, Q = TT::Q, GB = TT::GB, RB = TT::RB;
     uint32_t acc[GB][Q], y[GB][Q], x[16];
@@ -101,7 +104,7 @@ __device__ __forceinline__ void body_encode_tile(IO &io, int q, int p, uint32_t 
     for (uint32_t c = 0; c < chunks; ++c) {
         const uint32_t B = (c + 1) * T;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) x[j] = io.ld_in(c * T + q * 16, j);
+        for (int j = 0; j < 16; ++j) x[j] = (x0 && c == 0) ? x0[j] : io.ld_in(c * T + q * 16, j);
         run_levels<4, 0, 0, 4, true, false>(x, B + q * 16, tw);
         uint32_t *tb = tile + (NBUF == 2 ? (c & 1) * TT::WORDS : 0);
         if (NBUF == 1 && c > 0) __syncthreads();
