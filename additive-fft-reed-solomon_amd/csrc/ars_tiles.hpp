@@ -120,8 +120,10 @@ __device__ __forceinline__ void body_encode_tile(IO &io, int q, int p, uint32_t 
     }
 #pragma unroll
     for (int m = 0; m < GB; ++m) run_levels<RB, 4, 4, TL, false, true>(acc[m], 0u, tw);
+    // No barrier before put_b, even with one buffer: put_b writes exactly the
+    // words this lane's last get_b read (same index set), and one lane's LDS
+    // read and later write of a word are ordered.
     uint32_t *tb = tile + (NBUF == 2 ? (chunks & 1) * TT::WORDS : 0);
-    if (NBUF == 1) __syncthreads();
     TT::put_b(tb, acc, q, p);
     __syncthreads();
     TT::get_a(tb, x, q, p);
@@ -238,7 +240,9 @@ __device__ __forceinline__ void body_recover_tile(IO &io, const TABS &tabs, int 
     TT::get_b(t0, y, q, p);
 #pragma unroll
     for (int m = 0; m < GB; ++m) run_levels<RB, 4, 4, NL, true, true>(y[m], 0u, tw);
-    if constexpr (NBUF == 1) __syncthreads();
+    // put_b rewrites exactly the words this lane's get_b read: no barrier
+    // needed before it even when t1 == t0 (the one before the second put_b
+    // is needed: the derivative reads other lanes' words)
     TT::put_b(t1, y, q, p);
     __syncthreads();
     // formal derivative: bits 4.. from registers, bits 0..3 from LDS
