@@ -22,9 +22,11 @@ def ars():
 def test_every_variant_is_bit_exact(ars, quads, buffers):
     ctx = ars.Context(0)
     rng = np.random.default_rng(quads + buffers)
-    for n, T in [(64, 32), (128, 64), (256, 128), (256, 32), (128, 32)]:
+    # small shards, then one large enough for thousands of resident
+    # workgroups (the exchange barriers are exercised under full occupancy)
+    for n, T, L in [(64, 32, 640), (128, 64, 640), (256, 128, 640), (256, 32, 640), (128, 32, 640),
+                    (256, 128, 65536)]:
         k = n - T
-        L = 640
         tl, nl = T.bit_length() - 1, n.bit_length() - 1
         if tl >= 5:
             ctx.tune_set(ars.ARS_OP_ENCODE, tl, quads, buffers)
