@@ -235,8 +235,8 @@ static hipError_t enc_acc(bool al, bool acc, const uint8_t *msg, uint8_t *par, u
 hipError_t launch_encode_ref(int tl, bool al, bool acc, const uint8_t *msg, uint8_t *par,
                              uint32_t L, uint32_t chunks, const MulTab *tw, hipStream_t s,
                              int pq, int nbuf) {
-    if (pq == 0) pq = tl >= 8 ? 32 : 64;
-    if (nbuf == 0) nbuf = 2;
+    if (pq == 0) pq = default_pq(0, tl);
+    if (nbuf == 0) nbuf = default_nbuf(0, tl);
     if (!variant_ok(tl, pq, nbuf)) return hipErrorInvalidValue;
     switch (tl) {
         case 0: return enc_acc<0>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
@@ -307,8 +307,8 @@ static hipError_t rec_one(uint8_t *data, uint32_t L, const RecPrep *prep, const 
 
 hipError_t launch_recover(int nl, bool al, uint8_t *data, uint32_t L, const RecPrep *prep,
                           const MulTab *tw, hipStream_t s, int pq, int nbuf) {
-    if (pq == 0) pq = nl >= 8 ? 32 : 64;
-    if (nbuf == 0) nbuf = nl >= 8 ? 1 : 2;
+    if (pq == 0) pq = default_pq(1, nl);
+    if (nbuf == 0) nbuf = default_nbuf(1, nl);
     if (!variant_ok(nl, pq, nbuf)) return hipErrorInvalidValue;
 #define RC_CASE(K) \
     case K: return al ? rec_one<K, true>(data, L, prep, tw, s, pq, nbuf) : rec_one<K, false>(data, L, prep, tw, s, pq, nbuf);

@@ -21,6 +21,14 @@ struct PosArg {
 // exp[0..511], log[512..767], omega[768..1023].
 constexpr int kGtabBytes = 1024;
 
+// Default tile variant (quads per workgroup, LDS buffers) of an operation
+// (0 = encode, 1 = recover) on 2^tl points, used when the context holds no
+// tuned choice: 32 quads with one buffer for the largest transforms (128-point
+// encode, 256-point recover; the autotuner's pick there on an MI355X), 64
+// quads with two buffers below.
+constexpr int default_pq(int op, int tl) { return tl >= (op == 0 ? 7 : 8) ? 32 : 64; }
+constexpr int default_nbuf(int op, int tl) { return tl >= (op == 0 ? 7 : 8) ? 1 : 2; }
+
 // pq / nbuf select a tile variant for transforms of 32+ points (0 = default):
 // pq = column quads per workgroup (16, 32 or 64), nbuf = LDS exchange buffers (1 or 2).
 hipError_t launch_encode_ref(int tl, bool aligned, bool accumulate, const uint8_t *msg, uint8_t *par,

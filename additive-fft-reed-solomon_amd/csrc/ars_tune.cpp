@@ -95,8 +95,7 @@ int ars_autotune(ars_ctx *ctx, uint32_t n, uint32_t k, size_t L, ars_tune_result
                                  s, a, b);
                 (void)hipGetLastError();
                 if (us < 0) continue;
-                const bool is_default = op == 0 ? (v.pq == (tl >= 8 ? 32 : 64) && v.nbuf == 2)
-                                                : (v.pq == (tl >= 8 ? 32 : 64) && v.nbuf == (tl >= 8 ? 1 : 2));
+                const bool is_default = v.pq == ars::default_pq(op, tl) && v.nbuf == ars::default_nbuf(op, tl);
                 if (is_default) dflt = us;
                 if (best < 0 || us < best) {
                     best = us;
