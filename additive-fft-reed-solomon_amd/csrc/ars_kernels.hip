@@ -84,7 +84,7 @@ __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
     for (int j = 0; j < 16; ++j) x0[j] = io.ld_in((uint32_t)(q * 16), j);
     stage_twiddles<TT::THREADS>(tw, g_tw);
     __syncthreads();
-    body_encode_tile<TL, PQ, ACC, NBUF>(io, q, p, tile, ONE ? 1u : chunks, tw, x0);
+    body_encode_tile<TL, PQ, ACC, NBUF, BufIO<AL>, true>(io, q, p, tile, ONE ? 1u : chunks, tw, x0, g_tw);
 }
 
 template <int KL, bool AL>
