@@ -325,13 +325,43 @@ __device__ __forceinline__ void body_recompute_tile(IO &io, int q, int p, uint32
 }
 
 // ---------------------------------------------------------------------------
+// log_2 a in GF(2^8) / 0x11d (a != 0), at compile time.
+constexpr uint32_t gf_clog(uint32_t a) {
+    uint32_t x = 1, i = 0;
+    while (x != a) {
+        x <<= 1;
+        if (x & 0x100u) x ^= 0x11du;
+        ++i;
+    }
+    return i;
+}
+
+// log(v << 3k) for the entries v = 1..7 of the three parts k of a MulTab
+// (part 2 has entries 1..3 only).
+struct PartLogs {
+    uint32_t l[3][8];
+};
+constexpr PartLogs make_part_logs() {
+    PartLogs c{};
+    for (uint32_t v = 1; v < 8; ++v) {
+        c.l[0][v] = gf_clog(v);
+        c.l[1][v] = gf_clog(v << 3);
+        c.l[2][v] = v < 4 ? gf_clog(v << 6) : 0u;
+    }
+    return c;
+}
+constexpr PartLogs kPartLog = make_part_logs();
+
+// ---------------------------------------------------------------------------
 // Recover: per-job multiplier tables.  For shard i, Pi(omega_i) =
 // prod_{p in E} (omega_i + omega_p); erased shards get 1 / Pi'(omega_i) with
 // Pi'(omega_i) = prod_{p in E, p != i} (omega_i + omega_p) -- the
 // reference's `denoms` (rs/src/codec.rs:457-462).  One workgroup per job.
 __device__ __forceinline__ void recover_prep_body(RecPrep *__restrict__ out, uint32_t n, uint32_t e,
                                                   const uint8_t *pos, const uint8_t *__restrict__ gtab) {
-    __shared__ uint8_t s_exp[512], s_log[256], s_sub[256], s_er[256], s_pts[256];
+    __shared__ uint8_t s_exp[512], s_log[256], s_sub[256], s_er[256];
+    __shared__ uint32_t s_pts4[64];  // the e erased points, 4 per word
+    uint8_t *s_pts = reinterpret_cast<uint8_t *>(s_pts4);
     const uint32_t t = threadIdx.x;
     s_exp[t] = gtab[t];
     s_exp[t + 256] = gtab[t + 256];
@@ -346,20 +376,31 @@ __device__ __forceinline__ void recover_prep_body(RecPrep *__restrict__ out, uin
     __syncthreads();
     if (t < n) {
         const uint8_t x = s_sub[t];
+        // Sum of log(x + omega_p) over the erased p.  The terms depend on each
+        // other only through the add, so the loop is branch-free and unrolled:
+        // a thread keeps several LDS lookups in flight.  A zero difference (p = t)
+        // adds log[0] = 0 (the uploaded table's entry 0).  Points are read 4 per
+        // word (the word read is the same for every lane).
+        const uint32_t x4 = (uint32_t)x * 0x01010101u;
         uint32_t lsum = 0;
-        for (uint32_t i = 0; i < e; ++i) {
-            const uint8_t dlt = x ^ s_pts[i];
-            if (dlt) lsum += s_log[dlt];
+#pragma unroll 4
+        for (uint32_t i = 0; i < (e >> 2); ++i) {
+            const uint32_t d = x4 ^ s_pts4[i];
+            lsum += (uint32_t)s_log[d & 0xffu] + s_log[(d >> 8) & 0xffu] + s_log[(d >> 16) & 0xffu] +
+                    s_log[d >> 24];
         }
+        for (uint32_t i = e & ~3u; i < e; ++i) lsum += s_log[x ^ s_pts[i]];
         lsum %= 255u;
         const uint32_t er = s_er[t];
         const uint32_t lv = er ? (255u - lsum) % 255u : lsum;
-        uint8_t tb[3][8];
-        for (int v = 0; v < 8; ++v) {
-            const uint32_t m0 = (uint32_t)v, m1 = (uint32_t)v << 3, m2 = (uint32_t)v << 6;
-            tb[0][v] = m0 ? s_exp[lv + s_log[m0]] : 0;
-            tb[1][v] = m1 ? s_exp[lv + s_log[m1]] : 0;
-            tb[2][v] = (v < 4 && m2) ? s_exp[lv + s_log[m2 & 0xff]] : 0;
+        // table entry v of part k is c * (v << 3k) = exp[log c + log(v << 3k)];
+        // the second log is a compile-time constant
+        uint8_t tb[3][8] = {};
+#pragma unroll
+        for (int v = 1; v < 8; ++v) {
+            tb[0][v] = s_exp[lv + kPartLog.l[0][v]];
+            tb[1][v] = s_exp[lv + kPartLog.l[1][v]];
+            if (v < 4) tb[2][v] = s_exp[lv + kPartLog.l[2][v]];
         }
         MulTab m;
         for (int k = 0; k < 3; ++k) {
