@@ -23,11 +23,11 @@ constexpr int kGtabBytes = 1024;
 
 // Default tile variant (quads per workgroup, LDS buffers) of an operation
 // (0 = encode, 1 = recover) on 2^tl points, used when the context holds no
-// tuned choice: 32 quads with one buffer for the largest transforms (128-point
-// encode, 256-point recover; the autotuner's pick there on an MI355X), 64
-// quads with two buffers below.
+// tuned choice: 32 quads for the largest transforms (128-point encode with one
+// buffer, 256-point recover with two; the autotuner's picks there on an
+// MI355X), 64 quads with two buffers below.
 constexpr int default_pq(int op, int tl) { return tl >= (op == 0 ? 7 : 8) ? 32 : 64; }
-constexpr int default_nbuf(int op, int tl) { return tl >= (op == 0 ? 7 : 8) ? 1 : 2; }
+constexpr int default_nbuf(int op, int tl) { return (op == 0 && tl >= 7) ? 1 : 2; }
 
 // pq / nbuf select a tile variant for transforms of 32+ points (0 = default):
 // pq = column quads per workgroup (16, 32 or 64), nbuf = LDS exchange buffers (1 or 2).

@@ -226,6 +226,23 @@ __device__ __forceinline__ void body_recover_small(IO &io, const TABS &tabs, con
         if (tabs.erased(i)) io.st_out(0u, i, gmul4(x[i], tabs.tab(i)) ^ orig[i]);
 }
 
+// Low-bit part of the novel-basis formal derivative on 16 consecutive
+// points: d[j] = xor over bits b < 4 clear in j of x[j | 2^b].
+__device__ __forceinline__ void low_derivative(const uint32_t (&x)[16], uint32_t (&d)[16]) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        uint32_t a = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (!((j >> b) & 1)) a ^= x[j | (1 << b)];
+        d[j] = a;
+    }
+}
+
+// NBUF = 2 (two exchange buffers): the derivative's low-bit terms are taken
+// from the phase-A output (see below), which saves the LDS copy of D and its
+// cross-lane reads: two exchange barriers instead of four.  NBUF = 1 keeps the
+// direct form: D is copied to LDS and the low-bit terms are read across lanes.
 template <int NL, int PQ, int NBUF, class IO, class TABS>
 __device__ __forceinline__ void body_recover_tile(IO &io, const TABS &tabs, int q, int p, uint32_t *tile,
                                                   const MulTab *tw) {
@@ -247,34 +264,68 @@ __device__ __forceinline__ void body_recover_tile(IO &io, const TABS &tabs, int 
     TT::get_b(t0, y, q, p);
 #pragma unroll
     for (int m = 0; m < GB; ++m) run_levels<RB, 4, 4, NL, true, true>(y[m], 0u, tw);
-    // put_b rewrites exactly the words this lane's get_b read: no barrier
-    // needed before it even when t1 == t0 (the one before the second put_b
-    // is needed: the derivative reads other lanes' words)
-    TT::put_b(t1, y, q, p);
-    __syncthreads();
-    // formal derivative: bits 4.. from registers, bits 0..3 from LDS
+    if constexpr (NBUF == 1) {
+        // put_b rewrites exactly the words this lane's get_b read: no barrier
+        // needed before it
+        TT::put_b(t1, y, q, p);
+        __syncthreads();
+        // formal derivative: bits 4.. from registers, bits 0..3 from LDS
 #pragma unroll
-    for (int m = 0; m < GB; ++m) {
-        const int c = q * GB + m;
+        for (int m = 0; m < GB; ++m) {
+            const int c = q * GB + m;
 #pragma unroll
-        for (int h = 0; h < Q; ++h) {
-            uint32_t a = 0;
+            for (int h = 0; h < Q; ++h) {
+                uint32_t a = 0;
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const uint32_t v = t1[(h * 16 + (c | (1 << b))) * PQ + p];
-                a ^= ((c >> b) & 1) ? 0u : v;
+                for (int b = 0; b < 4; ++b) {
+                    const uint32_t v = t1[(h * 16 + (c | (1 << b))) * PQ + p];
+                    a ^= ((c >> b) & 1) ? 0u : v;
+                }
+#pragma unroll
+                for (int b = 0; b < RB; ++b)
+                    if (!((h >> b) & 1)) a ^= y[m][h | (1 << b)];
+                y[m][h] = a;
             }
-#pragma unroll
-            for (int b = 0; b < RB; ++b)
-                if (!((h >> b) & 1)) a ^= y[m][h | (1 << b)];
-            y[m][h] = a;
+            run_levels<RB, 4, 4, NL, false, true>(y[m], 0u, tw);
         }
-        run_levels<RB, 4, 4, NL, false, true>(y[m], 0u, tw);
+        __syncthreads();  // the derivative read other lanes' words of t0
+        TT::put_b(t0, y, q, p);
+        __syncthreads();
+        TT::get_a(t0, x, q, p);
+    } else {
+        // The formal derivative splits by index bits: D' = Dhi(D) + Dlo(D),
+        // Dlo the terms of bits 0..3 and Dhi those of bits 4...  Dlo acts on
+        // the low 4 index bits alike for every value of the high ones, and the
+        // phase-B levels (4..NL-1, beta = 0) act on the high bits with twiddles
+        // that do not depend on the low ones, so Dlo commutes with both phase-B
+        // transforms and, x being the phase-A output,
+        //   FFT_B(D') = FFT_B(Dhi(D)) + Dlo(FFT_B(IFFT_B(x))) = FFT_B(Dhi(D)) + Dlo(x).
+        // In the phase-A layout a lane holds the 16 shards q*16 + j, so Dlo(x)
+        // needs only the lane's own words.
+#pragma unroll
+        for (int m = 0; m < GB; ++m) {
+            // Dhi in place: ascending h only reads higher, still untouched, indices
+#pragma unroll
+            for (int h = 0; h < Q; ++h) {
+                uint32_t a = 0;
+#pragma unroll
+                for (int b = 0; b < RB; ++b)
+                    if (!((h >> b) & 1)) a ^= y[m][h | (1 << b)];
+                y[m][h] = a;
+            }
+            run_levels<RB, 4, 4, NL, false, true>(y[m], 0u, tw);
+        }
+        TT::put_b(t1, y, q, p);
+        __syncthreads();
+        // t0 still holds x (after put_a it was only read), so the lane's own
+        // words give Dlo(x) without keeping 16 more registers across phase B
+        uint32_t dlo[16];
+        TT::get_a(t0, x, q, p);
+        low_derivative(x, dlo);
+        TT::get_a(t1, x, q, p);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) x[j] ^= dlo[j];
     }
-    if constexpr (NBUF == 1) __syncthreads();
-    TT::put_b(t0, y, q, p);
-    __syncthreads();
-    TT::get_a(t0, x, q, p);
     run_levels<4, 0, 0, 4, false, false>(x, q * 16, tw);
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
