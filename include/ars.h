@@ -249,7 +249,8 @@ int ars_server_replay(ars_server *s, const ars_request *reqs, const double *arri
 /* ------------------------------------------------------------------------
  * Autotuner.  For transforms of 32+ points the fused kernels come in tile
  * variants: column quads per workgroup (16, 32 or 64) and LDS exchange buffers
- * (1: half the LDS, extra barriers; 2: one barrier per exchange).
+ * (1: half the LDS, extra barriers; 2: one barrier per exchange, and for
+ * recover the formal derivative's low-bit terms taken without an LDS copy).
  * ars_autotune times every variant for one (n, k, shard_len) on scratch
  * buffers and keeps the fastest in the context for that operation and
  * transform size.  ars_batch_autotune picks the persistent grid (workgroups
