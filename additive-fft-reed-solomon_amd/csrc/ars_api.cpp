@@ -63,6 +63,32 @@ int ars::check_code(uint32_t n, uint32_t k, size_t L) {
 namespace {
 using ars::check_code;
 bool aligned4(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; }
+
+// The plan buffer ars_recover keeps for `s`, created on the stream's first
+// call.  Work on one stream runs in issue order, so a call's plan kernel
+// overwrites the buffer only after the previous call's tile kernel has read
+// it; a stream handle is reused only after the stream it named was destroyed,
+// which HIP does once that stream's work has completed.  Keeping the buffer
+// saves the stream-ordered free of a per-call allocation, whose completion
+// marker leaves the GPU idle for about 6 us before the next kernel.  Returns
+// null (the caller then allocates on the stream) for hipStreamPerThread, one
+// handle that names a different stream on each host thread, for a stream
+// under graph capture, whose graph may be replayed on any stream, and once
+// the context holds kMaxPlanSlots buffers.
+ars::PlanSlot *plan_slot(ars_ctx *ctx, hipStream_t s) {
+    if (s == hipStreamPerThread) return nullptr;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
+    std::lock_guard<std::mutex> lk(ctx->plan_mu);
+    for (auto &p : ctx->plan_slots)
+        if (p->stream == s) return p.get();
+    if (ctx->plan_slots.size() >= ars::kMaxPlanSlots) return nullptr;
+    std::unique_ptr<ars::PlanSlot> slot(new (std::nothrow) ars::PlanSlot);
+    if (!slot || hipMalloc(reinterpret_cast<void **>(&slot->buf), sizeof(ars::RecPrep)) != hipSuccess) return nullptr;
+    slot->stream = s;
+    ctx->plan_slots.push_back(std::move(slot));
+    return ctx->plan_slots.back().get();
+}
 }  // namespace
 
 extern "C" {
@@ -109,6 +135,7 @@ int ars_destroy(ars_ctx *ctx) {
     ars::release_decode_state(ctx);
     hipFree(ctx->d_tw);
     hipFree(ctx->d_gtab);
+    for (auto &p : ctx->plan_slots) hipFree(p->buf);
     delete ctx;
     return ARS_OK;
 }
@@ -200,14 +227,24 @@ int ars_recover(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
+    const bool al = (L % 4 == 0) && aligned4(d_received);
+    const int nl = ilog2(n);
+    auto enqueue = [&](ars::RecPrep *prep) {
+        hipError_t le = ars::launch_recover_prep(prep, n, e_cnt, pa, ctx->d_gtab, s);
+        if (le == hipSuccess)
+            le = ars::launch_recover(nl, al, d_received, L, prep, ctx->d_tw, s, ctx->tune_pq[1][nl],
+                                     ctx->tune_nbuf[1][nl]);
+        return le;
+    };
+    if (ars::PlanSlot *slot = plan_slot(ctx, s)) {
+        std::lock_guard<std::mutex> lk(slot->mu);
+        if ((e = enqueue(slot->buf)) != hipSuccess) return hip_fail(e, "recover launch");
+        return ARS_TRUE;
+    }
     ars::RecPrep *prep = nullptr;
     if ((e = hipMallocAsync(reinterpret_cast<void **>(&prep), sizeof(ars::RecPrep), s)) != hipSuccess)
         return hip_fail(e, "hipMallocAsync");
-    const bool al = (L % 4 == 0) && aligned4(d_received);
-    e = ars::launch_recover_prep(prep, n, e_cnt, pa, ctx->d_gtab, s);
-    const int nl = ilog2(n);
-    if (e == hipSuccess)
-        e = ars::launch_recover(nl, al, d_received, L, prep, ctx->d_tw, s, ctx->tune_pq[1][nl], ctx->tune_nbuf[1][nl]);
+    e = enqueue(prep);
     hipError_t ef = hipFreeAsync(prep, s);
     if (e != hipSuccess) return hip_fail(e, "recover launch");
     if (ef != hipSuccess) return hip_fail(ef, "hipFreeAsync");

@@ -3,9 +3,25 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "ars_gf.hpp"
+#include "ars_kernels.hpp"
+
+namespace ars {
+// ars_recover's erasure plan buffer for one stream of a context.  `mu` is held
+// while a call enqueues its plan kernel and its tile kernel, so two host
+// threads on one stream cannot interleave the pair.
+struct PlanSlot {
+    hipStream_t stream = nullptr;
+    RecPrep *buf = nullptr;
+    std::mutex mu;
+};
+constexpr size_t kMaxPlanSlots = 64;  // streams per context with a kept plan buffer
+}  // namespace ars
 
 struct ars_ctx {
     int device = 0;
@@ -16,6 +32,10 @@ struct ars_ctx {
     // op 0 = encode (reference algorithm), 1 = recover
     uint8_t tune_pq[2][9] = {{0}};
     uint8_t tune_nbuf[2][9] = {{0}};
+    // ars_recover's plan buffers, one per stream the context has recovered on
+    // (at most kMaxPlanSlots; later streams allocate per call)
+    std::mutex plan_mu;
+    std::vector<std::unique_ptr<ars::PlanSlot>> plan_slots;
 };
 
 namespace ars {

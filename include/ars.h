@@ -20,8 +20,10 @@
  * All pointers named d_* are device pointers (HBM).  shard_len may be any
  * value >= 1 with n * shard_len < 4 GiB; shard_len % 4 == 0 with 4-byte
  * aligned pointers takes the dword path.  No entry point allocates on the hot path except
- * ars_recover (a small stream-ordered table, freed on the same stream) and
- * the *_host variants (staging buffers).
+ * ars_recover (a 6.4 KB erasure plan buffer, allocated on a context's first
+ * call on each stream and kept until ars_destroy; per call, stream-ordered,
+ * on hipStreamPerThread, under graph capture and past 64 streams) and the
+ * *_host variants (staging buffers).
  *
  * Return codes: >= 0 success (ars_recover: 1 = the reference's `true`,
  * 0 = `false` because more than T erasures), < 0 = ARS_E_* below, for inputs
@@ -79,7 +81,9 @@ int ars_encode(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *d_message,
  * (any order).  Writes every erased shard p; as in the reference, whatever p
  * held before is XORed into the result (zero-filled erasures come back as
  * the original shard).  Returns 1, 0 if e > n - k (received untouched), or
- * ARS_E_*.  Asynchronous on `stream`. */
+ * ARS_E_*.  Asynchronous on `stream`.  Calls on one stream, from any number
+ * of host threads, share the context's plan buffer for that stream; their
+ * kernels run in issue order. */
 int ars_recover(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_t shard_len,
                 const uint8_t *positions, uint32_t e, void *stream);
 
