@@ -154,6 +154,60 @@ def test_recover_edge_cases(ars):
         assert ei.value.code in (ars.ARS_E_INVAL, ars.ARS_E_UNSUPPORTED)
 
 
+def guarded(data, shift, pad=8):
+    """`data` on the device starting `shift` bytes past a 4-byte boundary,
+    with `pad` guard bytes of 0xA5 on both sides; returns (buffer, ptr)."""
+    buf = torch.full((len(data) + shift + pad + 4,), 0xA5, dtype=torch.uint8, device="cuda")
+    base = pad + shift
+    assert (buf.data_ptr() + pad) % 4 == 0
+    buf[base:base + len(data)] = dev(data)
+    return buf, buf.data_ptr() + base
+
+
+def unguard(buf, shift, size, pad=8):
+    h = host(buf)
+    base = pad + shift
+    assert np.all(h[:base] == 0xA5) and np.all(h[base + size:] == 0xA5), "write outside the buffer"
+    return h[base:base + size]
+
+
+@pytest.mark.parametrize("n,T", [(16, 8), (64, 32), (256, 128), (256, 16), (128, 64)])
+@pytest.mark.parametrize("shift", [1, 2, 3])
+def test_misaligned_buffers_match_oracle(ars, n, T, shift):
+    """Dword-multiple shard lengths on buffers that start off a 4-byte
+    boundary take the byte-wise I/O path; results and guard bytes are exact."""
+    k, L = n - T, 68
+    rng = np.random.default_rng(n * 7 + T * 3 + shift)
+    ctx = ars.Context(0)
+    msg = rng.integers(0, 256, k * L, dtype=np.uint8)
+    exp = O.encode_any(n, k, msg, L)
+    bm, pm = guarded(msg, shift)
+    bp, pp = guarded(np.zeros(T * L, np.uint8), (4 - shift) % 4)
+    ctx.encode(n, k, pm, pp, L)
+    assert np.array_equal(unguard(bp, (4 - shift) % 4, T * L), exp)
+    assert np.array_equal(unguard(bm, shift, k * L), msg)
+    cw = np.concatenate([exp, msg])
+    pos = rng.permutation(n)[:T].astype(np.uint8)
+    rx = cw.copy().reshape(n, L)
+    rx[pos] = 0
+    rx = rx.reshape(-1)
+    b, p = guarded(rx, shift)
+    assert ctx.recover(n, k, p, L, pos) == 1
+    assert np.array_equal(unguard(b, shift, n * L), cw)
+    plan = torch.empty(ars.lib().ars_recover_plan_size(), dtype=torch.uint8, device="cuda")
+    assert ctx.recover_plan(n, k, pos, plan.data_ptr()) == 1
+    b, p = guarded(rx, shift)
+    assert ctx.recover_with_plan(n, k, p, L, plan.data_ptr()) == 1
+    assert np.array_equal(unguard(b, shift, n * L), cw)
+    if n == 2 * T:
+        lost = cw.copy()
+        lost[T * L:] = 0
+        b, p = guarded(lost, shift)
+        ctx.recompute_data(n, k, p, L)
+        assert np.array_equal(unguard(b, shift, n * L), cw)
+    ctx.close()
+
+
 def test_recompute_data_matches_oracle(ars):
     rng = np.random.default_rng(2)
     for n in (2, 4, 8, 16, 32, 64, 128, 256):
