@@ -154,7 +154,7 @@ def test_recover_edge_cases(ars):
         assert ei.value.code in (ars.ARS_E_INVAL, ars.ARS_E_UNSUPPORTED)
 
 
-def guarded(data, shift, pad=8):
+def guarded(data, shift, pad=256):
     """`data` on the device starting `shift` bytes past a 4-byte boundary,
     with `pad` guard bytes of 0xA5 on both sides; returns (buffer, ptr)."""
     buf = torch.full((len(data) + shift + pad + 4,), 0xA5, dtype=torch.uint8, device="cuda")
@@ -164,7 +164,7 @@ def guarded(data, shift, pad=8):
     return buf, buf.data_ptr() + base
 
 
-def unguard(buf, shift, size, pad=8):
+def unguard(buf, shift, size, pad=256):
     h = host(buf)
     base = pad + shift
     assert np.all(h[:base] == 0xA5) and np.all(h[base + size:] == 0xA5), "write outside the buffer"
@@ -172,19 +172,22 @@ def unguard(buf, shift, size, pad=8):
 
 
 @pytest.mark.parametrize("n,T", [(16, 8), (64, 32), (256, 128), (256, 16), (128, 64)])
-@pytest.mark.parametrize("shift", [1, 2, 3])
-def test_misaligned_buffers_match_oracle(ars, n, T, shift):
-    """Dword-multiple shard lengths on buffers that start off a 4-byte
-    boundary take the byte-wise I/O path; results and guard bytes are exact."""
+@pytest.mark.parametrize("shift", [0, 1, 2, 3])
+def test_guarded_buffers_match_oracle(ars, n, T, shift):
+    """Shard length 68 (17 column quads: the last tile is partial) on buffers
+    that start `shift` bytes past a 4-byte boundary.  Shift 0 takes the dword
+    I/O path, shifts 1-3 the byte-wise one; results and the guard bytes on
+    both sides of every buffer are exact."""
     k, L = n - T, 68
     rng = np.random.default_rng(n * 7 + T * 3 + shift)
     ctx = ars.Context(0)
     msg = rng.integers(0, 256, k * L, dtype=np.uint8)
     exp = O.encode_any(n, k, msg, L)
     bm, pm = guarded(msg, shift)
-    bp, pp = guarded(np.zeros(T * L, np.uint8), (4 - shift) % 4)
+    ps = (4 - shift) % 4
+    bp, pp = guarded(np.zeros(T * L, np.uint8), ps)
     ctx.encode(n, k, pm, pp, L)
-    assert np.array_equal(unguard(bp, (4 - shift) % 4, T * L), exp)
+    assert np.array_equal(unguard(bp, ps, T * L), exp)
     assert np.array_equal(unguard(bm, shift, k * L), msg)
     cw = np.concatenate([exp, msg])
     pos = rng.permutation(n)[:T].astype(np.uint8)
