@@ -95,3 +95,19 @@ int main() {
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(bad)],
                        capture_output=True, text=True)
     assert r.returncode != 0 and "T >= 2" in r.stderr
+
+
+def test_codeword_size_limit_without_gpu():
+    """n * shard_len must stay below 4 GiB (32-bit buffer offsets): a larger
+    codeword is rejected with ARS_E_INVAL by the argument checks, before the
+    context is used or any HIP call is made."""
+    ars = _pkg.load()
+    if not os.path.exists(ars.LIB_PATH):
+        pytest.skip("libars.so not built")
+    lib = ars.lib()
+    bogus_ctx = 1  # never dereferenced: the size check fails first
+    L = 1 << 24    # 256 * 16 MiB = 4 GiB
+    assert lib.ars_encode(bogus_ctx, 256, 128, 1, 1, L, 0, None) == ars.ARS_E_INVAL
+    assert lib.ars_recover(bogus_ctx, 256, 128, 1, L, None, 0, None) == ars.ARS_E_INVAL
+    assert lib.ars_recompute_data(bogus_ctx, 256, 128, 1, L, None) == ars.ARS_E_INVAL
+    assert "4 GiB" in lib.ars_last_error().decode()

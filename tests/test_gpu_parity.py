@@ -289,3 +289,37 @@ def test_full_size_256_128_1mib(ars):
     got = host(cw)
     assert np.array_equal(got[:T * L], exp_par)
     assert np.array_equal(got[T * L:], msg)
+
+
+@pytest.mark.parametrize("L", [(1 << 24) - 64, (1 << 24) - 1])
+def test_largest_codeword_under_4gib(ars, L):
+    """(256,128) with n * shard_len just under the 4 GiB limit, on the dword
+    path (L % 4 == 0) and on the byte path.  Columns are independent, so the
+    encode is compared with the oracle on column slices at both ends of the
+    shards (the largest row and column offsets); recover of 128 erasures must
+    return the whole 4 GiB codeword."""
+    n, k = 256, 128
+    T = n - k
+    assert n * L < 1 << 32
+    ctx = ars.Context(0)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(L)
+    cw = torch.empty(n * L, dtype=torch.uint8, device="cuda")
+    cw[T * L:] = torch.randint(0, 256, (k * L,), dtype=torch.uint8, device="cuda", generator=g)
+    assert ctx.encode(n, k, cw.data_ptr() + T * L, cw.data_ptr(), L) == 0
+    torch.cuda.synchronize()
+    rows = cw.view(n, L)
+    w = 4099
+    for c0 in (0, L - w):
+        sl = rows[:, c0:c0 + w].cpu().numpy()
+        exp = O.encode(n, T, np.ascontiguousarray(sl[T:]).reshape(-1), w)
+        assert np.array_equal(np.ascontiguousarray(sl[:T]).reshape(-1), exp), c0
+    ref = cw.clone()
+    pos = np.random.default_rng(L).permutation(n)[:T].astype(np.uint8)
+    rows[torch.from_numpy(pos.astype(np.int64)).cuda()] = 0
+    assert ctx.recover(n, k, cw.data_ptr(), L, pos) == 1
+    torch.cuda.synchronize()
+    assert torch.equal(cw, ref)
+    del cw, ref, rows
+    torch.cuda.empty_cache()
+    ctx.close()
