@@ -75,6 +75,7 @@ SIGNATURES = {
     "ars_server_stats_get": (_i, [_vp, _vp]),
     "ars_server_stats_reset": (_i, [_vp]),
     "ars_server_replay": (_i, [_vp, _vp, _vp, _u32, _vp, _vp]),
+    "ars_plain_replay": (_i, [_vp, _vp, _vp, _u32, _vp, _sz, _vp, _vp, _vp]),
     "ars_autotune": (_i, [_vp, _u32, _u32, _sz, _vp, _vp, _vp]),
     "ars_tune_set": (_i, [_vp, _u32, _u32, _u32, _u32]),
     "ars_batch_autotune": (_i, [_vp, _vp, _u32, ctypes.POINTER(_u32), _vp]),
@@ -208,6 +209,17 @@ class Context:
         conv = lambda r: {f: getattr(r, f) for f, _ in TuneResult._fields_}
         return conv(er), conv(rr)
 
+    def plain_replay(self, reqs, arrivals, d_buffer, buffer_bytes, stream=None):
+        """ars_plain_replay: the trace served by one plain call per request on
+        one stream; returns (statuses, latencies in seconds)."""
+        count = len(arrivals)
+        arr = np.ascontiguousarray(np.asarray(arrivals, dtype=np.float64))
+        st = np.zeros(count, np.int32)
+        lat = np.zeros(count, np.float64)
+        _check(lib().ars_plain_replay(self.handle, reqs, arr.ctypes.data, count, d_buffer, buffer_bytes,
+                                      st.ctypes.data, lat.ctypes.data, _stream_ptr(stream)))
+        return st, lat
+
     def tune_set(self, op, log2_size, quads, buffers):
         _check(lib().ars_tune_set(self.handle, op, log2_size, quads, buffers))
 
@@ -300,7 +312,7 @@ class Batch:
 class ServerConfig(ctypes.Structure):
     """struct ars_server_config (include/ars.h)."""
     _fields_ = [("hbm_budget", ctypes.c_uint64), ("max_batch_jobs", ctypes.c_uint32),
-                ("pipeline_depth", ctypes.c_uint32)]
+                ("pipeline_depth", ctypes.c_uint32), ("direct_min_bytes", ctypes.c_uint64)]
 
 
 class Request(ctypes.Structure):
@@ -324,9 +336,9 @@ class ServerStats(ctypes.Structure):
 class Server:
     """Continuous-batching request loop (ars_server_*)."""
 
-    def __init__(self, hbm_budget=64 << 30, max_batch_jobs=1024, pipeline_depth=2, ctx=None):
+    def __init__(self, hbm_budget=64 << 30, max_batch_jobs=1024, pipeline_depth=2, direct_min_bytes=0, ctx=None):
         self.ctx = ctx or default_context(0)
-        cfg = ServerConfig(hbm_budget, max_batch_jobs, pipeline_depth)
+        cfg = ServerConfig(hbm_budget, max_batch_jobs, pipeline_depth, direct_min_bytes)
         h = ctypes.c_void_p()
         _check(lib().ars_server_create(self.ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h

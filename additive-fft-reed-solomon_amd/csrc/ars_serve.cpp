@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -33,6 +34,7 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr size_t kPoolAlign = 256;
+constexpr size_t kMinBatchJobs = 4;  // slots with fewer batchable jobs run them as plain calls
 
 // First-fit allocator over [0, cap) with coalescing frees.
 class Pool {
@@ -92,7 +94,8 @@ struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
     std::vector<Req *> reqs;
-    std::vector<ars_job> jobs;
+    std::vector<ars_job> jobs;    // run in the batched kernel
+    std::vector<ars_job> direct;  // run as plain calls (codewords of direct_min_bytes or more)
     bool busy = false;
 };
 
@@ -111,6 +114,7 @@ struct ars_server {
     std::vector<Slot> slots;
     std::thread disp;
     bool stop = false;
+    std::atomic<uint64_t> pending{0};  // submissions (and the stop request) the dispatcher has not yet seen
     uint64_t next_ticket = 1, outstanding = 0;
     Clock::time_point t0;
     // statistics since the last reset
@@ -127,10 +131,12 @@ double secs(const ars_server *s, Clock::time_point t) {
 
 void launch_slot(ars_server *s, Slot &sl) {
     // a request whose input copy cannot be queued fails with that error and
-    // stays out of the batch
-    std::vector<size_t> in_batch;
+    // stays out of the batch; codewords of direct_min_bytes or more run as
+    // plain calls after the batch, on the same stream
+    std::vector<size_t> in_batch, in_direct;
     in_batch.reserve(sl.reqs.size());
     sl.jobs.clear();
+    sl.direct.clear();
     for (size_t i = 0; i < sl.reqs.size(); ++i) {
         Req *q = sl.reqs[i];
         const ars_request &r = q->r;
@@ -157,24 +163,43 @@ void launch_slot(ars_server *s, Slot &sl) {
             q->status = ars::hip_error(e, "request input copy");
             continue;
         }
-        in_batch.push_back(i);
-        sl.jobs.push_back(j);
+        if ((uint64_t)r.n * r.shard_len >= s->cfg.direct_min_bytes) {
+            in_direct.push_back(i);
+            sl.direct.push_back(j);
+        } else {
+            in_batch.push_back(i);
+            sl.jobs.push_back(j);
+        }
     }
-    const int rc = sl.jobs.empty()
-                       ? ARS_OK
-                       : ars_batch_run(sl.batch, sl.jobs.data(), (uint32_t)sl.jobs.size(), 0, sl.stream);
-    for (size_t b = 0; b < in_batch.size(); ++b) {
-        Req *q = sl.reqs[in_batch[b]];
+    // a batch's fixed cost (planning, descriptor upload, its launches) pays
+    // off only over several jobs: fewer run as plain calls
+    if (!sl.jobs.empty() && sl.jobs.size() < kMinBatchJobs) {
+        in_direct.insert(in_direct.end(), in_batch.begin(), in_batch.end());
+        sl.direct.insert(sl.direct.end(), sl.jobs.begin(), sl.jobs.end());
+        in_batch.clear();
+        sl.jobs.clear();
+    }
+    const int rc_b = sl.jobs.empty()
+                         ? ARS_OK
+                         : ars_batch_run(sl.batch, sl.jobs.data(), (uint32_t)sl.jobs.size(), 0, sl.stream);
+    const int rc_d = sl.direct.empty()
+                         ? ARS_OK
+                         : ars_run_jobs_plain(s->ctx, sl.direct.data(), (uint32_t)sl.direct.size(), sl.stream);
+    auto finish_request = [&](Req *q, int32_t status) {
         const ars_request &r = q->r;
-        q->status = rc < 0 ? rc : sl.jobs[b].status;
-        if (!r.host_out || q->status < 0) continue;
+        q->status = status;
+        if (!r.host_out || q->status < 0) return;
         const size_t L = (size_t)r.shard_len;
         uint8_t *d = s->d_pool + q->off;
         hipError_t e = hipSuccess;
         if (r.op == ARS_OP_ENCODE) e = hipMemcpyAsync(r.host_out, d, (r.n - r.k) * L, hipMemcpyDeviceToHost, sl.stream);
         else if (q->status == ARS_TRUE) e = hipMemcpyAsync(r.host_out, d, r.n * L, hipMemcpyDeviceToHost, sl.stream);
         if (e != hipSuccess) q->status = ars::hip_error(e, "request output copy");
-    }
+    };
+    for (size_t b = 0; b < in_batch.size(); ++b)
+        finish_request(sl.reqs[in_batch[b]], rc_b < 0 ? rc_b : sl.jobs[b].status);
+    for (size_t b = 0; b < in_direct.size(); ++b)
+        finish_request(sl.reqs[in_direct[b]], rc_d < 0 ? rc_d : sl.direct[b].status);
     // the slot retires (and its pool space is reused) once the event has
     // completed; without a recorded event, wait here instead
     if (hipEventRecord(sl.ev, sl.stream) != hipSuccess) (void)hipStreamSynchronize(sl.stream);
@@ -194,24 +219,44 @@ void finish(ars_server *s, Req *q, Clock::time_point now) {
 
 void dispatcher(ars_server *s) {
     hipSetDevice(s->ctx->device);
-    std::unique_lock<std::mutex> lk(s->mu);
+    std::vector<char> done(s->slots.size(), 0);
     for (;;) {
-        bool progress = false;
-        // 1. retire finished batches
-        for (Slot &sl : s->slots) {
-            if (!sl.busy) continue;
-            if (hipEventQuery(sl.ev) != hipSuccess) continue;
-            const Clock::time_point now = Clock::now();
+        // 1. poll the slots' events without the lock (the slots belong to
+        //    this thread), so submitters are not held up meanwhile
+        bool any_done = false, busy = false;
+        for (size_t i = 0; i < s->slots.size(); ++i) {
+            done[i] = s->slots[i].busy && hipEventQuery(s->slots[i].ev) == hipSuccess;
+            any_done |= done[i] != 0;
+            busy |= s->slots[i].busy;
+        }
+        if (!any_done && s->pending.load(std::memory_order_acquire) == 0) {
+            if (busy) {
+                // work in flight: poll again at once.  A timed wait of a few
+                // microseconds oversleeps by the kernel's timer slack (tens of
+                // microseconds), which every request's latency would pay.
+                std::this_thread::yield();
+                continue;
+            }
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (s->waiting.empty() && s->admitted.empty() && !s->stop)
+                s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+        }
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->pending.store(0, std::memory_order_relaxed);
+        // 2. retire finished batches
+        const Clock::time_point now = Clock::now();
+        for (size_t i = 0; i < s->slots.size(); ++i) {
+            if (!done[i]) continue;
+            Slot &sl = s->slots[i];
             for (Req *q : sl.reqs) {
                 s->pool->release(q->off, q->bytes);
                 finish(s, q, now);
             }
             sl.reqs.clear();
             sl.busy = false;
-            progress = true;
             s->cv_done.notify_all();
         }
-        // 2. admission against the budget
+        // 3. admission against the budget
         for (auto it = s->waiting.begin(); it != s->waiting.end();) {
             Req *q = *it;
             size_t off;
@@ -219,34 +264,26 @@ void dispatcher(ars_server *s) {
                 q->off = off;
                 s->admitted.push_back(q);
                 it = s->waiting.erase(it);
-                progress = true;
             } else {
                 ++it;
             }
         }
-        // 3. launch on a free slot
-        if (!s->admitted.empty()) {
-            for (Slot &sl : s->slots) {
-                if (sl.busy) continue;
-                const size_t take = std::min<size_t>(s->admitted.size(), s->cfg.max_batch_jobs);
-                sl.reqs.assign(s->admitted.begin(), s->admitted.begin() + take);
-                s->admitted.erase(s->admitted.begin(), s->admitted.begin() + take);
-                s->batches++;
-                s->batch_jobs += take;
-                lk.unlock();
-                launch_slot(s, sl);
-                lk.lock();
-                progress = true;
-                break;
-            }
+        // 4. launch on the free slots
+        for (Slot &sl : s->slots) {
+            if (s->admitted.empty()) break;
+            if (sl.busy) continue;
+            const size_t take = std::min<size_t>(s->admitted.size(), s->cfg.max_batch_jobs);
+            sl.reqs.assign(s->admitted.begin(), s->admitted.begin() + take);
+            s->admitted.erase(s->admitted.begin(), s->admitted.begin() + take);
+            s->batches++;
+            s->batch_jobs += take;
+            lk.unlock();
+            launch_slot(s, sl);
+            lk.lock();
         }
-        bool busy = false;
-        for (const Slot &sl : s->slots) busy |= sl.busy;
-        if (s->stop && s->waiting.empty() && s->admitted.empty() && !busy) break;
-        if (!progress) {
-            if (busy) s->cv_work.wait_for(lk, std::chrono::microseconds(20));
-            else s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
-        }
+        bool any_busy = false;
+        for (const Slot &sl : s->slots) any_busy |= sl.busy;
+        if (s->stop && s->waiting.empty() && s->admitted.empty() && !any_busy) break;
     }
 }
 
@@ -266,6 +303,7 @@ int ars_server_create(ars_ctx *ctx, const ars_server_config *cfg, ars_server **o
     if (!s->cfg.hbm_budget) s->cfg.hbm_budget = 64ull << 30;
     if (!s->cfg.max_batch_jobs) s->cfg.max_batch_jobs = 1024;
     if (!s->cfg.pipeline_depth) s->cfg.pipeline_depth = 2;
+    if (!s->cfg.direct_min_bytes) s->cfg.direct_min_bytes = 16ull << 20;
     if ((e = hipMalloc(&s->d_pool, s->cfg.hbm_budget)) != hipSuccess) {
         delete s;
         return ars::hip_error(e, "hipMalloc(server pool)");
@@ -294,6 +332,7 @@ int ars_server_destroy(ars_server *s) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
+        s->pending.fetch_add(1, std::memory_order_release);
     }
     s->cv_work.notify_all();
     if (s->disp.joinable()) s->disp.join();
@@ -345,6 +384,7 @@ int ars_server_submit(ars_server *s, const ars_request *req, uint64_t *ticket) {
         s->outstanding++;
         if (s->first_arrival < 0) s->first_arrival = secs(s, q->t_submit);
         *ticket = q->ticket;
+        s->pending.fetch_add(1, std::memory_order_release);
     }
     s->cv_work.notify_one();
     return ARS_OK;
@@ -445,6 +485,82 @@ int ars_server_replay(ars_server *s, const ars_request *reqs, const double *arri
         if (latencies_s) latencies_s[i] = lat;
     }
     return rc;
+}
+
+int ars_plain_replay(ars_ctx *ctx, const ars_request *reqs, const double *arrival_s, uint32_t count,
+                     uint8_t *d_buffer, size_t buffer_bytes, int32_t *statuses, double *latencies_s,
+                     void *stream) {
+    if (!ctx || (count && (!reqs || !arrival_s || !d_buffer)))
+        return ars::set_error(ARS_E_NULL, "null context, trace or buffer");
+    for (uint32_t i = 0; i < count; ++i) {
+        const ars_request &r = reqs[i];
+        if (r.op != ARS_OP_ENCODE && r.op != ARS_OP_RECOVER) return ars::set_error(ARS_E_INVAL, "unknown op");
+        if ((uint64_t)r.n * r.shard_len > buffer_bytes)
+            return ars::set_error(ARS_E_INVAL, "a request's codeword does not fit the buffer");
+    }
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return ars::hip_error(e, "hipSetDevice");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // One event per request in flight, reused round-robin: request i records
+    // ring[i % R] only after the completion thread has passed request i - R.
+    constexpr uint32_t R = 256;
+    std::vector<hipEvent_t> ring(R, nullptr);
+    for (hipEvent_t &ev : ring)
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {
+            for (hipEvent_t x : ring)
+                if (x) hipEventDestroy(x);
+            return ars::hip_error(e, "hipEventCreate");
+        }
+    std::vector<int32_t> st(count, ARS_OK);
+    std::vector<double> lat(count, 0.0);
+    std::atomic<uint32_t> issued{0}, retired{0};
+    const Clock::time_point start = Clock::now();
+    auto due = [&](uint32_t i) {
+        return start + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival_s[i]));
+    };
+    std::thread completion([&] {
+        hipSetDevice(ctx->device);
+        for (uint32_t i = 0; i < count; ++i) {
+            while (issued.load(std::memory_order_acquire) <= i) std::this_thread::yield();
+            (void)hipEventSynchronize(ring[i % R]);
+            lat[i] = std::chrono::duration<double>(Clock::now() - due(i)).count();
+            retired.store(i + 1, std::memory_order_release);
+        }
+    });
+    // One stream runs its work in order, so every request can use the same
+    // device buffer: request i's input copy starts after request i - 1's
+    // output copy has finished.
+    for (uint32_t i = 0; i < count; ++i) {
+        const ars_request &r = reqs[i];
+        while (Clock::now() < due(i)) {
+            const auto left = due(i) - Clock::now();
+            if (left > std::chrono::microseconds(200)) std::this_thread::sleep_for(left - std::chrono::microseconds(100));
+        }
+        while (i >= R && retired.load(std::memory_order_acquire) < i - R + 1) std::this_thread::yield();
+        const size_t L = (size_t)r.shard_len, T = r.n - r.k;
+        int rc = ARS_OK;
+        if (r.host_in)
+            e = hipMemcpyAsync(r.op == ARS_OP_ENCODE ? d_buffer + T * L : d_buffer, r.host_in,
+                               (r.op == ARS_OP_ENCODE ? r.k : r.n) * L, hipMemcpyHostToDevice, s);
+        if (r.host_in && e != hipSuccess) rc = ars::hip_error(e, "request input copy");
+        if (rc == ARS_OK)
+            rc = r.op == ARS_OP_ENCODE ? ars_encode(ctx, r.n, r.k, d_buffer + T * L, d_buffer, L, r.flags, s)
+                                       : ars_recover(ctx, r.n, r.k, d_buffer, L, r.positions, r.e, s);
+        if (rc >= 0 && r.host_out && (r.op == ARS_OP_ENCODE || rc == ARS_TRUE)) {
+            e = hipMemcpyAsync(r.host_out, d_buffer, (r.op == ARS_OP_ENCODE ? T : r.n) * L, hipMemcpyDeviceToHost, s);
+            if (e != hipSuccess) rc = ars::hip_error(e, "request output copy");
+        }
+        st[i] = rc;
+        if ((e = hipEventRecord(ring[i % R], s)) != hipSuccess) (void)hipStreamSynchronize(s);
+        issued.store(i + 1, std::memory_order_release);
+    }
+    completion.join();
+    for (hipEvent_t ev : ring) hipEventDestroy(ev);
+    for (uint32_t i = 0; i < count; ++i) {
+        if (statuses) statuses[i] = st[i];
+        if (latencies_s) latencies_s[i] = lat[i];
+    }
+    return ARS_OK;
 }
 
 }  // extern "C"

@@ -259,42 +259,64 @@ def bench_batched(ars, torch, np, stream, count=4096, seed=44, reps=3):
     }
 
 
-def bench_served(ars, np, rates=(1000, 2000, 4000, 8000, 12000), per_rate=1500, seed=45):
+def bench_served(ars, torch, np, rates=(1000, 2000, 4000, 8000, 12000), per_rate=1500, seed=45):
     """Poisson arrivals, 70 % encode / 30 % recover, (n,k) ~ U{(64,32),(128,96),
     (256,128),(256,224)}, shard_len log-uniform 1 KiB-4 MiB (multiple of 64),
     recover e ~ U{1..T}; 64 GiB HBM admission budget; payloads are
-    device-resident (the pool slot's bytes), so latency is queueing + GPU."""
+    device-resident (the pool slot's bytes), so latency is queueing + GPU.
+    Each trace is then replayed through plain calls (ars_plain_replay: one
+    call per request, in arrival order on one stream, no batching) for the
+    plain-call latency and throughput beside the server's."""
     shapes = [(64, 32), (128, 96), (256, 128), (256, 224)]
     srv = ars.Server(hbm_budget=64 << 30, max_batch_jobs=1024, pipeline_depth=2)
+    ctx = ars.Context(0)
     out = []
     for ri, rate in enumerate(rates):
         rng = np.random.default_rng(seed + ri)
         reqs = (ars.Request * per_rate)()
         keep = []
+        alg = 0
+        biggest = 0
         for i in range(per_rate):
             n, k = shapes[rng.integers(4)]
             L = int(64 * max(1, round(2.0 ** rng.uniform(10, 22) / 64)))
+            biggest = max(biggest, n * L)
             if rng.random() < 0.7:
                 reqs[i] = ars.Server.request("encode", n, k, L)
+                alg += n * L
             else:
                 e = int(rng.integers(1, n - k + 1))
                 reqs[i] = ars.Server.request("recover", n, k, L, positions=rng.permutation(n)[:e], keep=keep)
+                alg += (n + e) * L
         arrivals = np.cumsum(rng.exponential(1.0 / rate, per_rate))
         srv.reset_stats()
         st, lat = srv.replay(reqs, arrivals)
         stt = srv.stats()
         span = stt["last_completion_s"] - stt["first_arrival_s"]
+        buf = torch.empty(biggest, dtype=torch.uint8, device="cuda")
+        stream = torch.cuda.Stream()
+        pst, plat = ctx.plain_replay(reqs, arrivals, buf.data_ptr(), biggest, stream)
+        pspan = float(np.max(arrivals + plat) - arrivals[0])
+        del buf
         out.append({"rate_req_per_s": rate, "requests": per_rate,
                     "ok": int((st >= 0).sum()),
                     "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
                     "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
                     "throughput_GBps": round(stt["algorithmic_bytes"] / span / 1e9, 1),
                     "achieved_req_per_s": round(per_rate / span, 1),
-                    "mean_batch_jobs": round(stt["mean_batch_jobs"], 2)})
+                    "mean_batch_jobs": round(stt["mean_batch_jobs"], 2),
+                    "plain": {"ok": int((pst >= 0).sum()),
+                              "p50_ms": round(float(np.percentile(plat, 50)) * 1e3, 3),
+                              "p99_ms": round(float(np.percentile(plat, 99)) * 1e3, 3),
+                              "throughput_GBps": round(alg / pspan / 1e9, 1),
+                              "achieved_req_per_s": round(per_rate / pspan, 1)}})
         print(f"served rate {rate}: {out[-1]}", file=sys.stderr, flush=True)
     srv.close()
+    ctx.close()
     return {"workload": "Poisson arrivals, 70% encode / 30% recover, (n,k) in {(64,32),(128,96),"
                         "(256,128),(256,224)}, L log-uniform 1 KiB-4 MiB, e~U{1..T}, 64 GiB budget",
+            "plain": "the same trace, one ars_encode / ars_recover call per request in arrival order on "
+                     "one stream (ars_plain_replay); latency = completion - arrival",
             "sweep": out}
 
 
@@ -566,7 +588,7 @@ def main():
         del plan
         out["batched"] = bench_batched(ars, torch, np, stream)
     if not args.no_served:
-        out["served"] = bench_served(ars, np)
+        out["served"] = bench_served(ars, torch, np)
     if not args.no_host:
         out["host_slices"] = bench_host(ars, torch, np, n, k, L, e)
         print(f"host slices: {out['host_slices']}", file=sys.stderr, flush=True)

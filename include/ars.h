@@ -206,6 +206,11 @@ typedef struct ars_server_config {
     uint64_t hbm_budget;       /* bytes of device pool for request codewords (0 = 64 GiB) */
     uint32_t max_batch_jobs;   /* jobs per launched batch (0 = 1024) */
     uint32_t pipeline_depth;   /* batches in flight at once, one stream each (0 = 2) */
+    uint64_t direct_min_bytes; /* requests whose codeword (n * shard_len) has at least this
+                                  many bytes run as plain ars_encode / ars_recover calls on
+                                  the batch's stream instead of in the batched kernel, where
+                                  launch overhead is negligible against their work
+                                  (0 = 16 MiB; UINT64_MAX = never) */
 } ars_server_config;
 
 typedef struct ars_request {
@@ -249,6 +254,19 @@ int ars_server_stats_reset(ars_server *s);
  * for all, and returns per-request statuses and latencies. */
 int ars_server_replay(ars_server *s, const ars_request *reqs, const double *arrival_s, uint32_t count,
                       int32_t *statuses, double *latencies_s);
+
+/* The plain-call baseline for ars_server_replay, with no counterpart in the
+ * reference: the same trace served by one call per request, in arrival order
+ * on `stream` (input copy if host_in, ars_encode / ars_recover, output copy if
+ * host_out), without batching.  Request i is issued at arrival_s[i] seconds
+ * after the call; a second thread timestamps completions.  d_buffer
+ * (buffer_bytes of device memory, at least n * shard_len of every request)
+ * holds one request's codeword at a time: the stream runs the requests in
+ * order.  statuses[i] = what ars_encode / ars_recover returned (or the copy's
+ * error); latencies_s[i] = completion - arrival, queueing included. */
+int ars_plain_replay(ars_ctx *ctx, const ars_request *reqs, const double *arrival_s, uint32_t count,
+                     uint8_t *d_buffer, size_t buffer_bytes, int32_t *statuses, double *latencies_s,
+                     void *stream);
 
 /* ------------------------------------------------------------------------
  * Autotuner.  For transforms of 32+ points the fused kernels come in tile

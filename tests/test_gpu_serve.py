@@ -1,5 +1,6 @@
 """GPU tests of the serving loop: results are the single-call results whether
-a request runs alone or in a mix submitted concurrently, the HBM budget
+a request runs alone or in a mix submitted concurrently (in the batched kernel,
+as plain calls, or both in one slot), the HBM budget
 admits requests as memory frees up, and over-budget requests are rejected."""
 import threading
 
@@ -38,7 +39,10 @@ def make_request_data(rng, n, k, L, op):
     return rx.reshape(-1).copy(), np.zeros(n * L, np.uint8), cw, pos
 
 
-def test_server_alone_equals_in_mix(ars):
+@pytest.mark.parametrize("direct_min_bytes", [0, 1, 128 << 10])
+def test_server_alone_equals_in_mix(ars, direct_min_bytes):
+    """Default threshold: every request here runs in the batched kernel;
+    1: every request runs as a plain call; 128 KiB: a slot mixes both."""
     rng = np.random.default_rng(45)
     specs = []
     for i in range(48):
@@ -46,7 +50,7 @@ def test_server_alone_equals_in_mix(ars):
         L = int(64 * rng.integers(1, 64))
         op = "encode" if rng.random() < 0.7 else "recover"
         specs.append((op, n, k, L) + make_request_data(rng, n, k, L, op))
-    srv = ars.Server(hbm_budget=256 << 20, max_batch_jobs=16, pipeline_depth=2)
+    srv = ars.Server(hbm_budget=256 << 20, max_batch_jobs=16, pipeline_depth=2, direct_min_bytes=direct_min_bytes)
     # alone: one request at a time
     alone = []
     for op, n, k, L, inp, out, exp, pos in specs:
@@ -149,3 +153,38 @@ def test_server_two_waiters_one_ticket(ars):
         assert [g[1] for g in got if g[0] == "ok"] == [0]
         assert np.array_equal(out, exp)
     srv.close()
+
+
+def test_plain_replay_matches_oracle(ars):
+    """ars_plain_replay, the served stream's plain-call baseline: a trace of
+    host-payload requests with arrivals that overlap is served one call per
+    request on one stream through one device buffer, and every output equals
+    the oracle; invalid requests report their error in `statuses`."""
+    rng = np.random.default_rng(47)
+    specs = []
+    for i in range(40):
+        n, k = SHAPES[rng.integers(len(SHAPES))]
+        L = int(64 * rng.integers(1, 64))
+        op = "encode" if rng.random() < 0.7 else "recover"
+        specs.append((op, n, k, L) + make_request_data(rng, n, k, L, op))
+    keep = []
+    outs = [np.zeros_like(sp[5]) for sp in specs]
+    reqs = (ars.Request * (len(specs) + 1))()
+    for i, sp in enumerate(specs):
+        reqs[i] = ars.Server.request(sp[0], sp[1], sp[2], sp[3], sp[7], host_in=sp[4], host_out=outs[i], keep=keep)
+    reqs[len(specs)] = ars.Server.request("recover", 64, 32, 64, [3, 3], keep=keep)  # duplicate position
+    arrivals = np.concatenate([np.sort(rng.uniform(0, 0.004, len(specs))), [0.005]])
+    biggest = max(sp[1] * sp[3] for sp in specs)
+    buf = torch.empty(biggest, dtype=torch.uint8, device="cuda")
+    ctx = ars.Context(0)
+    st, lat = ctx.plain_replay(reqs, arrivals, buf.data_ptr(), biggest)
+    for i, sp in enumerate(specs):
+        assert st[i] == (0 if sp[0] == "encode" else 1), i
+        assert lat[i] > 0
+        assert np.array_equal(outs[i], sp[6]), (i, sp[:4])
+    assert st[len(specs)] == ars.ARS_E_DUPLICATE
+    # a request larger than the buffer is rejected before anything runs
+    with pytest.raises(ars.ArsError) as ei:
+        ctx.plain_replay(reqs, arrivals, buf.data_ptr(), biggest - 1)
+    assert ei.value.code == ars.ARS_E_INVAL
+    ctx.close()
