@@ -210,7 +210,9 @@ typedef struct ars_server_config {
                                   many bytes run as plain ars_encode / ars_recover calls on
                                   the batch's stream instead of in the batched kernel, where
                                   launch overhead is negligible against their work
-                                  (0 = 16 MiB; UINT64_MAX = never) */
+                                  (0 = 16 MiB; UINT64_MAX = never).  A batch with fewer
+                                  than 4 jobs left for the kernel runs them as plain
+                                  calls too: its fixed cost would exceed theirs. */
 } ars_server_config;
 
 typedef struct ars_request {

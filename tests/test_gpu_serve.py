@@ -41,8 +41,9 @@ def make_request_data(rng, n, k, L, op):
 
 @pytest.mark.parametrize("direct_min_bytes", [0, 1, 128 << 10])
 def test_server_alone_equals_in_mix(ars, direct_min_bytes):
-    """Default threshold: every request here runs in the batched kernel;
-    1: every request runs as a plain call; 128 KiB: a slot mixes both."""
+    """Alone, a request is a slot of one job and runs as a plain call; in the
+    mix, slots of up to 16 jobs run in the batched kernel at the default
+    threshold, as plain calls with threshold 1, and as both at 128 KiB."""
     rng = np.random.default_rng(45)
     specs = []
     for i in range(48):
