@@ -31,6 +31,8 @@ struct ars_batch {
     std::vector<ars::BJob> jobs;
     std::vector<ars::BPrep> preps;
     std::vector<uint8_t> pos;
+    // outgrown arenas and staging buffers, freed by ars_batch_destroy
+    std::vector<uint8_t *> retired_arenas, retired_stages;
 };
 
 namespace {
@@ -77,11 +79,14 @@ int capture_graph(ars_batch *b) {
 
 int ensure_capacity(ars_batch *b, size_t arena_bytes, size_t stage_bytes) {
     hipError_t e;
+    // An outgrown buffer may still be read by work in flight: a previous
+    // run's kernels (on any stream) or its upload.  It is kept until
+    // ars_batch_destroy rather than freed here, since a free synchronises the
+    // whole device and would stall every stream's work.  Capacities grow by
+    // half, so what is kept stays below twice the final size.
     if (arena_bytes > b->arena_cap) {
-        // the previous arena may still be read by a run in flight
-        if ((e = hipDeviceSynchronize()) != hipSuccess) return ars::hip_error(e, "hipDeviceSynchronize");
-        destroy_graph(b);
-        hipFree(b->d_arena);
+        destroy_graph(b);  // captured with the old arena's address
+        if (b->d_arena) b->retired_arenas.push_back(b->d_arena);
         b->d_arena = nullptr;
         b->arena_cap = 0;
         const size_t cap = arena_bytes + arena_bytes / 2;
@@ -89,9 +94,7 @@ int ensure_capacity(ars_batch *b, size_t arena_bytes, size_t stage_bytes) {
         b->arena_cap = cap;
     }
     if (stage_bytes > b->stage_cap) {
-        if (b->upload_done && (e = hipEventSynchronize(b->upload_done)) != hipSuccess)
-            return ars::hip_error(e, "hipEventSynchronize");
-        hipHostFree(b->h_stage);
+        if (b->h_stage) b->retired_stages.push_back(b->h_stage);
         b->h_stage = nullptr;
         b->stage_cap = 0;
         const size_t cap = stage_bytes + stage_bytes / 2;
@@ -128,6 +131,19 @@ int ars_batch_create(ars_ctx *ctx, uint32_t max_jobs, ars_batch **out) {
     b->prep_grid = 256;
     const int bpc = ars::batch_blocks_per_cu();
     b->grid = (uint32_t)(bpc * (ctx->num_cus > 0 ? ctx->num_cus : 256));
+    // size the arena and its staging mirror up front for up to 4096 jobs of a
+    // few work items each, so that a serving loop's first large batches do
+    // not grow them (and recapture the graph) while other work is in flight
+    const size_t jobs = std::min<size_t>(b->max_jobs, 4096);
+    const size_t per_job = sizeof(ars::BJob) + sizeof(ars::BPrep) + 4 * (sizeof(ars::BItem) + sizeof(ars::BSeg)) + 256;
+    const size_t stage = align_up(sizeof(ars::BHeader)) + jobs * per_job;
+    int rc = ensure_capacity(b, stage + jobs * sizeof(ars::RecPrep), stage);
+    // capture (and instantiate) the graph now rather than on the first run
+    if (rc == ARS_OK) rc = capture_graph(b);
+    if (rc) {
+        ars_batch_destroy(b);
+        return rc;
+    }
     *out = b;
     return ARS_OK;
 }
@@ -143,6 +159,8 @@ int ars_batch_destroy(ars_batch *b) {
     if (b->run_done) hipEventDestroy(b->run_done);
     hipFree(b->d_arena);
     hipHostFree(b->h_stage);
+    for (uint8_t *p : b->retired_arenas) hipFree(p);
+    for (uint8_t *p : b->retired_stages) hipHostFree(p);
     delete b;
     return ARS_OK;
 }

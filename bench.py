@@ -320,6 +320,60 @@ def bench_served(ars, torch, np, rates=(1000, 2000, 4000, 8000, 12000), per_rate
             "sweep": out}
 
 
+def bench_served_small(ars, torch, np, rates=(20000, 50000, 100000, 200000), per_rate=4000, seed=48):
+    """The served loop on small codewords, where launch cost matters: Poisson
+    arrivals of the batched workload's shapes (n ~ U{32,64,128,256}, k = n/2,
+    shard_len log-uniform 64 B-64 KiB, multiple of 64), 70 % encode / 30 %
+    recover with e ~ U{1..T}, device-resident payloads; the server against
+    the same trace through plain calls (ars_plain_replay)."""
+    srv = ars.Server(hbm_budget=64 << 30, max_batch_jobs=1024, pipeline_depth=2)
+    ctx = ars.Context(0)
+    out = []
+    for ri, rate in enumerate(rates):
+        rng = np.random.default_rng(seed + ri)
+        reqs = (ars.Request * per_rate)()
+        keep = []
+        alg = biggest = 0
+        for i in range(per_rate):
+            n = int(2 ** rng.integers(5, 9))
+            k = n // 2
+            L = int(64 * max(1, round(2.0 ** rng.uniform(6, 16) / 64)))
+            biggest = max(biggest, n * L)
+            if rng.random() < 0.7:
+                reqs[i] = ars.Server.request("encode", n, k, L)
+                alg += n * L
+            else:
+                e = int(rng.integers(1, n - k + 1))
+                reqs[i] = ars.Server.request("recover", n, k, L, positions=rng.permutation(n)[:e], keep=keep)
+                alg += (n + e) * L
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, per_rate))
+        srv.reset_stats()
+        st, lat = srv.replay(reqs, arrivals)
+        stt = srv.stats()
+        span = stt["last_completion_s"] - stt["first_arrival_s"]
+        buf = torch.empty(biggest, dtype=torch.uint8, device="cuda")
+        pst, plat = ctx.plain_replay(reqs, arrivals, buf.data_ptr(), biggest, torch.cuda.Stream())
+        pspan = float(np.max(arrivals + plat) - arrivals[0])
+        del buf
+        out.append({"rate_req_per_s": rate, "requests": per_rate, "ok": int((st >= 0).sum()),
+                    "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                    "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                    "throughput_GBps": round(stt["algorithmic_bytes"] / span / 1e9, 1),
+                    "achieved_req_per_s": round(per_rate / span, 1),
+                    "mean_batch_jobs": round(stt["mean_batch_jobs"], 2),
+                    "plain": {"ok": int((pst >= 0).sum()),
+                              "p50_ms": round(float(np.percentile(plat, 50)) * 1e3, 3),
+                              "p99_ms": round(float(np.percentile(plat, 99)) * 1e3, 3),
+                              "throughput_GBps": round(alg / pspan / 1e9, 1),
+                              "achieved_req_per_s": round(per_rate / pspan, 1)}})
+        print(f"served small rate {rate}: {out[-1]}", file=sys.stderr, flush=True)
+    srv.close()
+    ctx.close()
+    return {"workload": "Poisson arrivals, 70% encode / 30% recover, n in {32,64,128,256}, k=n/2, "
+                        "L log-uniform 64 B-64 KiB, e~U{1..T}; server vs the same trace through plain calls",
+            "sweep": out}
+
+
 def bench_host(ars, torch, np, n, k, L, e, reps=5, seed=46):
     """The host-slice entry points (the Rust binding's path): the same
     (n, k, L) codeword in pageable host memory, streamed through the GPU in
@@ -589,6 +643,7 @@ def main():
         out["batched"] = bench_batched(ars, torch, np, stream)
     if not args.no_served:
         out["served"] = bench_served(ars, torch, np)
+        out["served_small"] = bench_served_small(ars, torch, np)
     if not args.no_host:
         out["host_slices"] = bench_host(ars, torch, np, n, k, L, e)
         print(f"host slices: {out['host_slices']}", file=sys.stderr, flush=True)
