@@ -229,6 +229,12 @@ int ars_recover(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_
     hipStream_t s = static_cast<hipStream_t>(stream);
     const bool al = (L % 4 == 0) && aligned4(d_received);
     const int nl = ilog2(n);
+    // Launches of at most one workgroup per compute unit compute the plan in
+    // the recover kernel itself: one launch instead of two.
+    e = ars::launch_recover_fused(nl, al, d_received, L, pa, e_cnt, ctx->d_gtab, ctx->d_tw, s,
+                                  (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256));
+    if (e == hipSuccess) return ARS_TRUE;
+    if (e != hipErrorNotSupported) return hip_fail(e, "recover launch");
     auto enqueue = [&](ars::RecPrep *prep) {
         hipError_t le = ars::launch_recover_prep(prep, n, e_cnt, pa, ctx->d_gtab, s);
         if (le == hipSuccess)

@@ -158,6 +158,53 @@ __global__ void __launch_bounds__((Tile<NL, PQ>::THREADS))
     body_recover_tile<NL, PQ, NBUF>(io, tabs, q, p, tile, tw);
 }
 
+// Recover with the erasure plan computed by each workgroup itself, for
+// launches of at most one workgroup per compute unit: every workgroup then
+// runs in the launch's first wave, so the plan costs one workgroup's time
+// instead of a second launch (the plan kernel) ahead of this one.  The tile
+// form keeps the plan's scratch in its tile buffer, idle until the first
+// exchange.
+template <int NL, bool AL>
+__global__ void __launch_bounds__(256) k_recover_small_fused(uint8_t *__restrict__ data, uint32_t L, PosArg pos,
+                                                             uint32_t e, const uint8_t *__restrict__ gtab,
+                                                             const MulTab *__restrict__ g_tw) {
+    constexpr int N = 1 << NL;
+    __shared__ MulTab tw[kTwiddleEntries];
+    __shared__ MulTab s_tab[N];
+    __shared__ __align__(4) uint8_t s_er[(N + 3) / 4 * 4];
+    __shared__ __align__(4) uint8_t s_scratch[1536];
+    stage_twiddles<256>(tw, g_tw);
+    recover_plan<256>(PlanToLds{s_tab, s_er}, N, e, pos.p, gtab,
+                      PlanScratch{s_scratch, s_scratch + 512, s_scratch + 768, s_scratch + 1024,
+                                  reinterpret_cast<uint32_t *>(s_scratch + 1280)});
+    __syncthreads();
+    BufIO<AL> io = make_bufio<AL>(data, N * L, data, N * L, blockIdx.x * blockDim.x + threadIdx.x, L);
+    LdsTabs tabs{s_tab, s_er};
+    body_recover_small<NL>(io, tabs, tw);
+}
+
+template <int NL, int PQ, bool AL, int NBUF>
+__global__ void __launch_bounds__((Tile<NL, PQ>::THREADS))
+    k_recover_tile_fused(uint8_t *__restrict__ data, uint32_t L, PosArg pos, uint32_t e,
+                         const uint8_t *__restrict__ gtab, const MulTab *__restrict__ g_tw) {
+    using TT = Tile<NL, PQ>;
+    constexpr int N = TT::N;
+    __shared__ uint32_t tile[NBUF * TT::WORDS];
+    __shared__ MulTab tw[kTwiddleEntries];
+    __shared__ MulTab s_tab[N];
+    __shared__ __align__(4) uint8_t s_er[N];
+    static_assert(NBUF * TT::WORDS * 4 >= 1536, "the plan's scratch must fit the tile buffer");
+    stage_twiddles<TT::THREADS>(tw, g_tw);
+    uint8_t *sc = reinterpret_cast<uint8_t *>(tile);
+    recover_plan<TT::THREADS>(PlanToLds{s_tab, s_er}, N, e, pos.p, gtab,
+                              PlanScratch{sc, sc + 512, sc + 768, sc + 1024, reinterpret_cast<uint32_t *>(sc + 1280)});
+    __syncthreads();  // the plan is in s_tab / s_er, and the tile buffer is free again
+    const int p = threadIdx.x % PQ, q = tile_q<PQ>();
+    BufIO<AL> io = make_bufio<AL>(data, N * L, data, N * L, blockIdx.x * PQ + p, L);
+    LdsTabs tabs{s_tab, s_er};
+    body_recover_tile<NL, PQ, NBUF>(io, tabs, q, p, tile, tw);
+}
+
 template <int TL, bool AL>
 __global__ void __launch_bounds__(256) k_recompute_small(uint8_t *__restrict__ data, uint32_t L,
                                                          const MulTab *__restrict__ g_tw) {
@@ -317,6 +364,37 @@ hipError_t launch_recover(int nl, bool al, uint8_t *data, uint32_t L, const RecP
         default: return hipErrorInvalidValue;
     }
 #undef RC_CASE
+}
+
+template <int NL, bool AL>
+static hipError_t rec_fused_one(uint8_t *data, uint32_t L, const PosArg &pos, uint32_t e, const uint8_t *gtab,
+                                const MulTab *tw, hipStream_t s, uint32_t max_groups) {
+    const uint64_t quads = ((uint64_t)L + 3) / 4;
+    if constexpr (NL <= 4) {
+        const uint32_t grid = ceil_div(quads, 256);
+        if (grid > max_groups) return hipErrorNotSupported;
+        hipLaunchKernelGGL((k_recover_small_fused<NL, AL>), dim3(grid), dim3(256), 0, s, data, L, pos, e, gtab, tw);
+    } else {
+        constexpr int PQ = default_pq(1, NL), NB = default_nbuf(1, NL);
+        const uint32_t grid = ceil_div(quads, PQ);
+        if (grid > max_groups) return hipErrorNotSupported;
+        hipLaunchKernelGGL((k_recover_tile_fused<NL, PQ, AL, NB>), dim3(grid), dim3(Tile<NL, PQ>::THREADS), 0, s,
+                           data, L, pos, e, gtab, tw);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_recover_fused(int nl, bool al, uint8_t *data, uint32_t L, const PosArg &pos, uint32_t e,
+                                const uint8_t *gtab, const MulTab *tw, hipStream_t s, uint32_t max_groups) {
+#define RF_CASE(K)                                                                                  \
+    case K:                                                                                         \
+        return al ? rec_fused_one<K, true>(data, L, pos, e, gtab, tw, s, max_groups)               \
+                  : rec_fused_one<K, false>(data, L, pos, e, gtab, tw, s, max_groups);
+    switch (nl) {
+        RF_CASE(1) RF_CASE(2) RF_CASE(3) RF_CASE(4) RF_CASE(5) RF_CASE(6) RF_CASE(7) RF_CASE(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef RF_CASE
 }
 
 template <int TL, bool AL>

@@ -40,6 +40,11 @@ hipError_t launch_recover_prep(RecPrep *out, uint32_t n, uint32_t e, const PosAr
                                const uint8_t *gtab, hipStream_t s);
 hipError_t launch_recover(int nl, bool aligned, uint8_t *data, uint32_t L, const RecPrep *prep,
                           const MulTab *tw, hipStream_t s, int pq = 0, int nbuf = 0);
+// Recover in one launch, each workgroup computing the plan itself (the
+// default tile variant), when the launch has at most max_groups workgroups;
+// hipErrorNotSupported (nothing launched) otherwise.
+hipError_t launch_recover_fused(int nl, bool aligned, uint8_t *data, uint32_t L, const PosArg &pos, uint32_t e,
+                                const uint8_t *gtab, const MulTab *tw, hipStream_t s, uint32_t max_groups);
 hipError_t launch_recompute(int tl, bool aligned, uint8_t *data, uint32_t L, const MulTab *tw,
                             hipStream_t s);
 

@@ -407,26 +407,56 @@ constexpr PartLogs kPartLog = make_part_logs();
 // Recover: per-job multiplier tables.  For shard i, Pi(omega_i) =
 // prod_{p in E} (omega_i + omega_p); erased shards get 1 / Pi'(omega_i) with
 // Pi'(omega_i) = prod_{p in E, p != i} (omega_i + omega_p) -- the
-// reference's `denoms` (rs/src/codec.rs:457-462).  One workgroup per job.
-__device__ __forceinline__ void recover_prep_body(RecPrep *__restrict__ out, uint32_t n, uint32_t e,
-                                                  const uint8_t *pos, const uint8_t *__restrict__ gtab) {
-    __shared__ uint8_t s_exp[512], s_log[256], s_sub[256], s_er[256];
-    __shared__ uint32_t s_pts4[64];  // the e erased points, 4 per word
-    uint8_t *s_pts = reinterpret_cast<uint8_t *>(s_pts4);
-    const uint32_t t = threadIdx.x;
-    s_exp[t] = gtab[t];
-    s_exp[t + 256] = gtab[t + 256];
-    s_log[t] = gtab[512 + t];
-    s_sub[t] = gtab[768 + t];
-    s_er[t] = 0;
-    __syncthreads();
-    if (t < e) {
-        s_er[pos[t]] = 1;
-        s_pts[t] = s_sub[pos[t]];
+// reference's `denoms` (rs/src/codec.rs:457-462).
+
+// LDS scratch of the plan computation: exp[512], log[256], the points
+// omega_i (sub[256]), erased flags er[256] and the e erased points, 4 per
+// word (pts4[64]); 1.5 KiB in all.
+struct PlanScratch {
+    uint8_t *exp, *log, *sub, *er;
+    uint32_t *pts4;
+};
+
+// Where the plan goes: a RecPrep in global memory (the plan kernel) ...
+struct PlanToGlobal {
+    RecPrep *p;
+    __device__ __forceinline__ void set(uint32_t i, const MulTab &m, uint8_t er) const {
+        p->tab[i] = m;
+        p->erased[i] = er;
+    }
+};
+// ... or a recover workgroup's own LDS tables.
+struct PlanToLds {
+    MulTab *tab;
+    uint8_t *er;
+    __device__ __forceinline__ void set(uint32_t i, const MulTab &m, uint8_t e) const {
+        tab[i] = m;
+        er[i] = e;
+    }
+};
+
+// The plan of one erasure pattern (n points, e erased at pos[0..e)), by a
+// workgroup of THREADS threads; ends with the outputs written by each thread
+// (the caller synchronises before reading them).
+template <int THREADS, class OUT>
+__device__ __forceinline__ void recover_plan(const OUT &out, uint32_t n, uint32_t e, const uint8_t *pos,
+                                             const uint8_t *__restrict__ gtab, const PlanScratch &sc) {
+    uint8_t *s_pts = reinterpret_cast<uint8_t *>(sc.pts4);
+    for (uint32_t t = threadIdx.x; t < 256u; t += THREADS) {
+        sc.exp[t] = gtab[t];
+        sc.exp[t + 256] = gtab[t + 256];
+        sc.log[t] = gtab[512 + t];
+        sc.sub[t] = gtab[768 + t];
+        sc.er[t] = 0;
     }
     __syncthreads();
-    if (t < n) {
-        const uint8_t x = s_sub[t];
+    for (uint32_t t = threadIdx.x; t < e; t += THREADS) {
+        sc.er[pos[t]] = 1;
+        s_pts[t] = sc.sub[pos[t]];
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < n; t += THREADS) {
+        const uint8_t x = sc.sub[t];
         // Sum of log(x + omega_p) over the erased p.  The terms depend on each
         // other only through the add, so the loop is branch-free and unrolled:
         // a thread keeps several LDS lookups in flight.  A zero difference (p = t)
@@ -436,31 +466,38 @@ __device__ __forceinline__ void recover_prep_body(RecPrep *__restrict__ out, uin
         uint32_t lsum = 0;
 #pragma unroll 4
         for (uint32_t i = 0; i < (e >> 2); ++i) {
-            const uint32_t d = x4 ^ s_pts4[i];
-            lsum += (uint32_t)s_log[d & 0xffu] + s_log[(d >> 8) & 0xffu] + s_log[(d >> 16) & 0xffu] +
-                    s_log[d >> 24];
+            const uint32_t d = x4 ^ sc.pts4[i];
+            lsum += (uint32_t)sc.log[d & 0xffu] + sc.log[(d >> 8) & 0xffu] + sc.log[(d >> 16) & 0xffu] +
+                    sc.log[d >> 24];
         }
-        for (uint32_t i = e & ~3u; i < e; ++i) lsum += s_log[x ^ s_pts[i]];
+        for (uint32_t i = e & ~3u; i < e; ++i) lsum += sc.log[x ^ s_pts[i]];
         lsum %= 255u;
-        const uint32_t er = s_er[t];
+        const uint32_t er = sc.er[t];
         const uint32_t lv = er ? (255u - lsum) % 255u : lsum;
         // table entry v of part k is c * (v << 3k) = exp[log c + log(v << 3k)];
         // the second log is a compile-time constant
         uint8_t tb[3][8] = {};
 #pragma unroll
         for (int v = 1; v < 8; ++v) {
-            tb[0][v] = s_exp[lv + kPartLog.l[0][v]];
-            tb[1][v] = s_exp[lv + kPartLog.l[1][v]];
-            if (v < 4) tb[2][v] = s_exp[lv + kPartLog.l[2][v]];
+            tb[0][v] = sc.exp[lv + kPartLog.l[0][v]];
+            tb[1][v] = sc.exp[lv + kPartLog.l[1][v]];
+            if (v < 4) tb[2][v] = sc.exp[lv + kPartLog.l[2][v]];
         }
         MulTab m;
         for (int k = 0; k < 3; ++k) {
             m.w[2 * k] = tb[k][0] | (tb[k][1] << 8) | (tb[k][2] << 16) | ((uint32_t)tb[k][3] << 24);
             m.w[2 * k + 1] = tb[k][4] | (tb[k][5] << 8) | (tb[k][6] << 16) | ((uint32_t)tb[k][7] << 24);
         }
-        out->tab[t] = m;
-        out->erased[t] = (uint8_t)er;
+        out.set(t, m, (uint8_t)er);
     }
+}
+
+// One workgroup of 256 threads per job, writing the plan to global memory.
+__device__ __forceinline__ void recover_prep_body(RecPrep *__restrict__ out, uint32_t n, uint32_t e,
+                                                  const uint8_t *pos, const uint8_t *__restrict__ gtab) {
+    __shared__ uint8_t s_exp[512], s_log[256], s_sub[256], s_er[256];
+    __shared__ uint32_t s_pts4[64];
+    recover_plan<256>(PlanToGlobal{out}, n, e, pos, gtab, PlanScratch{s_exp, s_log, s_sub, s_er, s_pts4});
 }
 
 // ---------------------------------------------------------------------------

@@ -323,3 +323,29 @@ def test_largest_codeword_under_4gib(ars, L):
     del cw, ref, rows
     torch.cuda.empty_cache()
     ctx.close()
+
+
+@pytest.mark.parametrize("n,T", [(2, 1), (16, 8), (32, 16), (64, 32), (128, 64), (256, 128), (256, 16)])
+def test_fused_and_split_recover_agree(ars, n, T):
+    """ars_recover computes the erasure plan inside the recover kernel when
+    its launch has at most one workgroup per compute unit, and launches the
+    plan kernel first otherwise.  On both sides of that threshold (and on the
+    byte path) the result equals the oracle's, garbage in the erased rows
+    included."""
+    k = n - T
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    cols = 1024 if n <= 16 else (128 if n == 256 else 256)  # byte columns per workgroup
+    rng = np.random.default_rng(n * 5 + T)
+    ctx = ars.Context(0)
+    for L in (cols * cus, cols * cus + 4, cols * cus + 3):
+        cw = make_cw(n, k, L, rng)
+        pos = rng.permutation(n)[:T].astype(np.uint8)
+        rx = cw.copy().reshape(n, L)
+        rx[pos] = rng.integers(0, 256, (T, L), dtype=np.uint8)
+        rx = rx.reshape(-1)
+        exp = rx.copy()
+        assert O.recover(n, T, exp, L, pos) == 1
+        d = dev(rx)
+        assert ctx.recover(n, k, d.data_ptr(), L, pos) == 1
+        assert np.array_equal(host(d), exp), (n, T, L)
+    ctx.close()
