@@ -22,8 +22,10 @@
  * aligned pointers takes the dword path.  No entry point allocates on the hot path except
  * ars_recover (a 6.4 KB erasure plan buffer, allocated on a context's first
  * call on each stream and kept until ars_destroy; per call, stream-ordered,
- * on hipStreamPerThread, under graph capture and past 64 streams) and the
- * *_host variants (staging buffers).
+ * on hipStreamPerThread, under graph capture and past 64 streams; launches of
+ * at most one workgroup per compute unit need none: their workgroups compute
+ * the plan themselves, in one launch) and the *_host variants (staging
+ * buffers).
  *
  * Return codes: >= 0 success (ars_recover: 1 = the reference's `true`,
  * 0 = `false` because more than T erasures), < 0 = ARS_E_* below, for inputs
