@@ -19,6 +19,15 @@ HIP_STREAM_PER_THREAD = 2  # hipStreamPerThread
 MAX_PLAN_SLOTS = 64        # streams per context with a kept plan buffer
 
 
+def split_len(n):
+    """A shard length whose recover launch has more workgroups than the GPU
+    has compute units, so ars_recover takes the plan-buffer path (smaller
+    launches compute the plan inside the recover kernel and need no buffer)."""
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    cols = 1024 if n <= 16 else (128 if n == 256 else 256)  # byte columns per workgroup
+    return cols * cus + 64
+
+
 @pytest.fixture(scope="module")
 def ars():
     if not torch.cuda.is_available():
@@ -52,7 +61,8 @@ def check(jobs):
 
 def test_back_to_back_on_one_stream(ars):
     ctx = ars.Context(0)
-    n, k, L = 256, 128, 4096
+    n, k = 256, 128
+    L = split_len(n)
     jobs = make_jobs(n, k, L, 12, 1)
     s = torch.cuda.Stream()
     for d, pos, _ in jobs:
@@ -64,7 +74,8 @@ def test_back_to_back_on_one_stream(ars):
 
 def test_interleaved_streams(ars):
     ctx = ars.Context(0)
-    n, k, L = 128, 64, 2048
+    n, k = 128, 64
+    L = split_len(n)
     jobs = make_jobs(n, k, L, 16, 2)
     streams = [torch.cuda.Stream() for _ in range(3)] + [torch.cuda.current_stream()]
     for i, (d, pos, _) in enumerate(jobs):
@@ -75,7 +86,8 @@ def test_interleaved_streams(ars):
 
 def test_two_threads_one_stream(ars):
     ctx = ars.Context(0)
-    n, k, L = 256, 128, 1024
+    n, k = 256, 128
+    L = split_len(n)
     jobs = make_jobs(n, k, L, 24, 3)
     s = torch.cuda.Stream()
     rcs = []
@@ -98,7 +110,8 @@ def test_two_threads_one_stream(ars):
 def test_per_thread_stream_and_many_streams(ars):
     hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime torch and the library share
     ctx = ars.Context(0)
-    n, k, L = 64, 32, 512
+    n, k = 64, 32
+    L = split_len(n)
     count = MAX_PLAN_SLOTS + 8
     jobs = make_jobs(n, k, L, count + 2, 4)
     raw = []
