@@ -34,7 +34,8 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr size_t kPoolAlign = 256;
-constexpr size_t kMinBatchJobs = 4;  // slots with fewer batchable jobs run them as plain calls
+constexpr size_t kMinBatchJobs = 4;    // slots with fewer batchable jobs run them as plain calls
+constexpr size_t kIdleBatchJobs = 16;  // ... and, while no other slot is busy, slots with fewer than this
 
 // First-fit allocator over [0, cap) with coalescing frees.
 class Pool {
@@ -129,7 +130,7 @@ double secs(const ars_server *s, Clock::time_point t) {
     return std::chrono::duration<double>(t - s->t0).count();
 }
 
-void launch_slot(ars_server *s, Slot &sl) {
+void launch_slot(ars_server *s, Slot &sl, bool loaded) {
     // a request whose input copy cannot be queued fails with that error and
     // stays out of the batch; codewords of direct_min_bytes or more run as
     // plain calls after the batch, on the same stream
@@ -172,8 +173,11 @@ void launch_slot(ars_server *s, Slot &sl) {
         }
     }
     // a batch's fixed cost (planning, descriptor upload, its launches) pays
-    // off only over several jobs: fewer run as plain calls
-    if (!sl.jobs.empty() && sl.jobs.size() < kMinBatchJobs) {
+    // off only over several jobs: fewer run as plain calls.  With no other
+    // slot busy the GPU would otherwise idle through that fixed cost, so the
+    // bar is higher; under load, batching keeps the launches few.
+    const size_t min_jobs = loaded ? kMinBatchJobs : kIdleBatchJobs;
+    if (!sl.jobs.empty() && sl.jobs.size() < min_jobs) {
         in_direct.insert(in_direct.end(), in_batch.begin(), in_batch.end());
         sl.direct.insert(sl.direct.end(), sl.jobs.begin(), sl.jobs.end());
         in_batch.clear();
@@ -220,6 +224,11 @@ void finish(ars_server *s, Req *q, Clock::time_point now) {
 void dispatcher(ars_server *s) {
     hipSetDevice(s->ctx->device);
     std::vector<char> done(s->slots.size(), 0);
+    // after work, keep polling this long before sleeping on the condition
+    // variable: waking a sleeping thread costs tens of microseconds, which a
+    // request arriving shortly after the last one would otherwise pay
+    const auto linger = std::chrono::microseconds(500);
+    Clock::time_point last_work = Clock::now();
     for (;;) {
         // 1. poll the slots' events without the lock (the slots belong to
         //    this thread), so submitters are not held up meanwhile
@@ -230,10 +239,11 @@ void dispatcher(ars_server *s) {
             busy |= s->slots[i].busy;
         }
         if (!any_done && s->pending.load(std::memory_order_acquire) == 0) {
-            if (busy) {
-                // work in flight: poll again at once.  A timed wait of a few
-                // microseconds oversleeps by the kernel's timer slack (tens of
-                // microseconds), which every request's latency would pay.
+            if (busy || Clock::now() - last_work < linger) {
+                // work in flight (or just done): poll again at once.  A timed
+                // wait of a few microseconds oversleeps by the kernel's timer
+                // slack (tens of microseconds), which every request's latency
+                // would pay.
                 std::this_thread::yield();
                 continue;
             }
@@ -242,7 +252,7 @@ void dispatcher(ars_server *s) {
                 s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
         }
         std::unique_lock<std::mutex> lk(s->mu);
-        s->pending.store(0, std::memory_order_relaxed);
+        if (s->pending.exchange(0, std::memory_order_relaxed) != 0 || any_done) last_work = Clock::now();
         // 2. retire finished batches
         const Clock::time_point now = Clock::now();
         for (size_t i = 0; i < s->slots.size(); ++i) {
@@ -277,8 +287,10 @@ void dispatcher(ars_server *s) {
             s->admitted.erase(s->admitted.begin(), s->admitted.begin() + take);
             s->batches++;
             s->batch_jobs += take;
+            bool loaded = false;
+            for (const Slot &o : s->slots) loaded |= o.busy;
             lk.unlock();
-            launch_slot(s, sl);
+            launch_slot(s, sl, loaded);
             lk.lock();
         }
         bool any_busy = false;
