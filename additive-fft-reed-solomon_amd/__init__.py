@@ -312,7 +312,8 @@ class Batch:
 class ServerConfig(ctypes.Structure):
     """struct ars_server_config (include/ars.h)."""
     _fields_ = [("hbm_budget", ctypes.c_uint64), ("max_batch_jobs", ctypes.c_uint32),
-                ("pipeline_depth", ctypes.c_uint32), ("direct_min_bytes", ctypes.c_uint64)]
+                ("pipeline_depth", ctypes.c_uint32), ("direct_min_bytes", ctypes.c_uint64),
+                ("min_batch_jobs", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class Request(ctypes.Structure):
@@ -336,9 +337,10 @@ class ServerStats(ctypes.Structure):
 class Server:
     """Continuous-batching request loop (ars_server_*)."""
 
-    def __init__(self, hbm_budget=64 << 30, max_batch_jobs=1024, pipeline_depth=2, direct_min_bytes=0, ctx=None):
+    def __init__(self, hbm_budget=64 << 30, max_batch_jobs=1024, pipeline_depth=2, direct_min_bytes=0,
+                 min_batch_jobs=0, ctx=None):
         self.ctx = ctx or default_context(0)
-        cfg = ServerConfig(hbm_budget, max_batch_jobs, pipeline_depth, direct_min_bytes)
+        cfg = ServerConfig(hbm_budget, max_batch_jobs, pipeline_depth, direct_min_bytes, min_batch_jobs, 0)
         h = ctypes.c_void_p()
         _check(lib().ars_server_create(self.ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h

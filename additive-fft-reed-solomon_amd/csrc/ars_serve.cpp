@@ -34,8 +34,11 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr size_t kPoolAlign = 256;
-constexpr size_t kMinBatchJobs = 4;    // slots with fewer batchable jobs run them as plain calls
-constexpr size_t kIdleBatchJobs = 16;  // ... and, while no other slot is busy, slots with fewer than this
+// Default batch thresholds (ars_server_config::min_batch_jobs = 0): slots
+// with fewer batchable jobs run them as plain calls, the second bound while
+// no other slot is busy.
+constexpr size_t kMinBatchJobs = 4;
+constexpr size_t kIdleBatchJobs = 16;
 
 // First-fit allocator over [0, cap) with coalescing frees.
 class Pool {
@@ -176,7 +179,7 @@ void launch_slot(ars_server *s, Slot &sl, bool loaded) {
     // off only over several jobs: fewer run as plain calls.  With no other
     // slot busy the GPU would otherwise idle through that fixed cost, so the
     // bar is higher; under load, batching keeps the launches few.
-    const size_t min_jobs = loaded ? kMinBatchJobs : kIdleBatchJobs;
+    const size_t min_jobs = s->cfg.min_batch_jobs ? s->cfg.min_batch_jobs : (loaded ? kMinBatchJobs : kIdleBatchJobs);
     if (!sl.jobs.empty() && sl.jobs.size() < min_jobs) {
         in_direct.insert(in_direct.end(), in_batch.begin(), in_batch.end());
         sl.direct.insert(sl.direct.end(), sl.jobs.begin(), sl.jobs.end());

@@ -212,9 +212,12 @@ typedef struct ars_server_config {
                                   many bytes run as plain ars_encode / ars_recover calls on
                                   the batch's stream instead of in the batched kernel, where
                                   launch overhead is negligible against their work
-                                  (0 = 16 MiB; UINT64_MAX = never).  A batch with fewer
-                                  than 4 jobs left for the kernel runs them as plain
-                                  calls too: its fixed cost would exceed theirs. */
+                                  (0 = 16 MiB; UINT64_MAX = never) */
+    uint32_t min_batch_jobs;   /* a batch with fewer jobs left for the kernel runs them as
+                                  plain calls too, its fixed cost exceeding theirs (0 = 4
+                                  while another batch is in flight, 16 on an otherwise idle
+                                  GPU; 1 = always use the kernel) */
+    uint32_t reserved;         /* 0 */
 } ars_server_config;
 
 typedef struct ars_request {

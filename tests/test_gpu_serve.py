@@ -1,6 +1,6 @@
 """GPU tests of the serving loop: results are the single-call results whether
 a request runs alone or in a mix submitted concurrently (in the batched kernel,
-as plain calls, or both in one slot), the HBM budget
+as plain calls, both in one slot, or by the default rules), the HBM budget
 admits requests as memory frees up, and over-budget requests are rejected."""
 import threading
 
@@ -39,11 +39,11 @@ def make_request_data(rng, n, k, L, op):
     return rx.reshape(-1).copy(), np.zeros(n * L, np.uint8), cw, pos
 
 
-@pytest.mark.parametrize("direct_min_bytes", [0, 1, 128 << 10])
-def test_server_alone_equals_in_mix(ars, direct_min_bytes):
-    """Alone, a request is a slot of one job and runs as a plain call; in the
-    mix, slots of up to 16 jobs run in the batched kernel at the default
-    threshold, as plain calls with threshold 1, and as both at 128 KiB."""
+@pytest.mark.parametrize("direct_min_bytes,min_batch_jobs", [(2 ** 64 - 1, 1), (1, 0), (128 << 10, 1), (0, 0)])
+def test_server_alone_equals_in_mix(ars, direct_min_bytes, min_batch_jobs):
+    """Each request alone and in a mix submitted from 4 threads: every job in
+    the batched kernel (even a slot of one); every job as a plain call; a
+    slot split by codeword size between the two; the default rules."""
     rng = np.random.default_rng(45)
     specs = []
     for i in range(48):
@@ -51,7 +51,8 @@ def test_server_alone_equals_in_mix(ars, direct_min_bytes):
         L = int(64 * rng.integers(1, 64))
         op = "encode" if rng.random() < 0.7 else "recover"
         specs.append((op, n, k, L) + make_request_data(rng, n, k, L, op))
-    srv = ars.Server(hbm_budget=256 << 20, max_batch_jobs=16, pipeline_depth=2, direct_min_bytes=direct_min_bytes)
+    srv = ars.Server(hbm_budget=256 << 20, max_batch_jobs=16, pipeline_depth=2, direct_min_bytes=direct_min_bytes,
+                     min_batch_jobs=min_batch_jobs)
     # alone: one request at a time
     alone = []
     for op, n, k, L, inp, out, exp, pos in specs:
