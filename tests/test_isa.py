@@ -80,10 +80,11 @@ def test_tile_kernels_address_lds_directly(asm):
 def test_default_kernels_do_not_spill(asm):
     # the variants the launchers pick without tuning: PQ 32 / NBUF 1 for the
     # 128-point encode, PQ 32 / NBUF 2 for the 256-point recover, PQ 64 / NBUF 2
-    # below
+    # below; the one-launch recovers use the same variants
     want = [r"k_encode_tileILi7ELi32ELb1ELb0ELi1ELb1E", r"k_encode_tileILi6ELi64ELb1ELb0ELi2E",
             r"k_recover_tileILi8ELi32ELb1ELi2E", r"k_recover_tileILi7ELi64ELb1ELi2E",
-            r"k_syndrome_tile", r"k_decode_columns"]
+            r"k_recover_tile_fusedILi8ELi32ELb1ELi2E", r"k_recover_tile_fusedILi5ELi64ELb0ELi2E",
+            r"k_recover_small_fusedILi4ELb0E", r"k_syndrome_tile", r"k_decode_columns"]
     seen = {}
     for src, fns in asm.items():
         for name, body in fns.items():
