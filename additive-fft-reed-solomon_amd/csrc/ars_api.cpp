@@ -208,7 +208,8 @@ int ars_recover_with_plan(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_recei
     const bool al = (L % 4 == 0) && aligned4(d_received);
     const int nl = ilog2(n);
     e = ars::launch_recover(nl, al, d_received, L, static_cast<const ars::RecPrep *>(d_plan),
-                            ctx->d_tw, static_cast<hipStream_t>(stream), ctx->tune_pq[1][nl], ctx->tune_nbuf[1][nl]);
+                            ctx->d_tw, static_cast<hipStream_t>(stream), ctx->tune_pq[1][nl], ctx->tune_nbuf[1][nl],
+                            (uint32_t)ctx->num_cus);
     if (e != hipSuccess) return hip_fail(e, "recover launch");
     return ARS_TRUE;
 }
@@ -239,7 +240,7 @@ int ars_recover(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_
         hipError_t le = ars::launch_recover_prep(prep, n, e_cnt, pa, ctx->d_gtab, s);
         if (le == hipSuccess)
             le = ars::launch_recover(nl, al, d_received, L, prep, ctx->d_tw, s, ctx->tune_pq[1][nl],
-                                     ctx->tune_nbuf[1][nl]);
+                                     ctx->tune_nbuf[1][nl], (uint32_t)ctx->num_cus);
         return le;
     };
     if (ars::PlanSlot *slot = plan_slot(ctx, s)) {

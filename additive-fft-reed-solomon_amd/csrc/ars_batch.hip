@@ -28,6 +28,7 @@ namespace ars {
 // Packed items: each lane addresses its own codeword as a 32-bit offset
 // from the item's base pointers, through one scalar buffer resource.
 struct PackIO {
+    static constexpr bool kUniformGroups = false;
     Rows in, out;
     uint32_t vin, vout, L, nb;
     __device__ __forceinline__ uint32_t ld_in(uint32_t g, int j) const {
@@ -38,6 +39,9 @@ struct PackIO {
     }
     __device__ __forceinline__ void st_out(uint32_t g, int j, uint32_t v) const {
         if (nb) __builtin_amdgcn_raw_buffer_store_b32(v, out.r, (int)(vout + (g + (uint32_t)j) * L), 0, 0);
+    }
+    __device__ __forceinline__ void xor_out(uint32_t g, int j, uint32_t v) const {
+        if (nb) ars_buffer_atomic_xor((int)v, out.r, (int)(vout + (g + (uint32_t)j) * L), 0, 0);
     }
 };
 
@@ -156,7 +160,10 @@ __device__ __forceinline__ void run_body(const BItem &it, IO &io, int q, int p, 
         if constexpr (TL <= 7) body_encode_kpow2_tile<TL, PQ, 1>(io, q, p, tile, it.arg & 0xffffu, it.arg >> 16, tw);
     } else {
         LdsTabs tabs{s_tab + si * (1u << TL), s_er + si * (1u << TL)};
-        body_recover_tile<TL, PQ, 1>(io, tabs, q, p, tile, tw);
+        const Flags16 f = group_flags<false>(tabs, q);
+        uint32_t x[16];
+        recover_rows_in<false>(io, f, q, x);
+        body_recover_tile1<TL, PQ>(io, tabs, f, q, p, tile, tw, x, [] {});
     }
 }
 
@@ -248,7 +255,7 @@ __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per
     __shared__ uint32_t tile[8192];  // 2^TL shards x batch_lanes(TL) quads, every TL >= 5
     __shared__ MulTab tw[kTwiddleEntries];
     __shared__ MulTab s_tab[kBatchPlanSlots];
-    __shared__ uint8_t s_er[kBatchPlanSlots];
+    __shared__ __align__(16) uint8_t s_er[kBatchPlanSlots];
     __shared__ uint32_t s_item;
     BHeader *h = reinterpret_cast<BHeader *>(arena);
     const BItem *items = reinterpret_cast<const BItem *>(arena + h->off_items);

@@ -122,6 +122,12 @@ struct Rows {
     __amdgpu_buffer_rsrc_t r;
 };
 
+// mem ^= v for one dword through a buffer resource; the L2 performs the
+// read-modify-write and nothing is returned (buffer_atomic_xor).  Bound to
+// the LLVM intrinsic directly: clang has no builtin for the xor form.
+__device__ int ars_buffer_atomic_xor(int v, __amdgpu_buffer_rsrc_t r, int voff, int soff, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.atomic.xor.i32");
+
 __device__ __forceinline__ Rows make_rows(const void *base, uint32_t bytes) {
     Rows w;
     w.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);

@@ -205,6 +205,77 @@ __global__ void __launch_bounds__((Tile<NL, PQ>::THREADS))
     body_recover_tile<NL, PQ, NBUF>(io, tabs, q, p, tile, tw);
 }
 
+// Recover of dword-addressable codewords (one exchange buffer, see
+// body_recover_tile1), capped at 128 VGPRs (4 waves per SIMD; uncapped, the
+// compiler takes 157 for 256 points): each workgroup stages the tables once and walks the
+// tiles blockIdx.x, blockIdx.x + gridDim.x, ...  The next tile's input rows
+// are requested once the current tile is back in the phase-A layout, so their
+// latency overlaps the phase-A FFT and the output of the current one.
+template <int NL, int PQ, bool OOB>
+__device__ __forceinline__ void recover_stream(uint8_t *data, uint32_t L, uint32_t tiles, const MulTab *s_tab,
+                                               const uint8_t *s_er, uint32_t *tile, const MulTab *tw) {
+    using TT = Tile<NL, PQ>;
+    constexpr uint32_t N = TT::N;
+    const int p = threadIdx.x % PQ, q = tile_q<PQ>();
+    const LdsTabs tabs{s_tab, s_er};
+    const Flags16 f = group_flags<PQ % 64 == 0>(tabs, q);  // the same as UNI below
+    uint32_t t = blockIdx.x;
+    constexpr bool UNI = PQ % 64 == 0;
+    BufIO<true, UNI> io = make_bufio<true, UNI>(data, N * L, data, N * L, t * PQ + p, L);
+    uint32_t rows[16];
+    recover_rows_in<OOB>(io, f, q, rows);
+    for (;;) {
+        const uint32_t tn = t + gridDim.x;
+        // the next tile's addressing is formed only where its rows are
+        // requested, so it is not live across phase B
+        body_recover_tile1<NL, PQ>(io, tabs, f, q, p, tile, tw, rows, [&] {
+            if (tn < tiles)
+                recover_rows_in<OOB>(make_bufio<true, UNI>(data, N * L, data, N * L, tn * PQ + p, L), f, q, rows);
+        });
+        if (tn >= tiles) break;
+        io = make_bufio<true, UNI>(data, N * L, data, N * L, tn * PQ + p, L);
+        t = tn;
+        __syncthreads();  // every lane's get_a of this tile is done before the next put_a
+    }
+}
+
+template <int NL, int PQ, bool OOB>
+__global__ void __launch_bounds__((Tile<NL, PQ>::THREADS)) __attribute__((amdgpu_waves_per_eu(4)))
+    k_recover_stream(uint8_t *__restrict__ data, uint32_t L, uint32_t tiles, const RecPrep *__restrict__ prep,
+                     const MulTab *__restrict__ g_tw) {
+    using TT = Tile<NL, PQ>;
+    constexpr int N = TT::N;
+    __shared__ uint32_t tile[TT::WORDS];
+    __shared__ MulTab tw[kTwiddleEntries];
+    __shared__ MulTab s_tab[N];
+    __shared__ __align__(16) uint8_t s_er[N];
+    stage_twiddles<TT::THREADS>(tw, g_tw);
+    stage_prep<TT::THREADS, N>(s_tab, s_er, prep);
+    __syncthreads();
+    recover_stream<NL, PQ, OOB>(data, L, tiles, s_tab, s_er, tile, tw);
+}
+
+// The same with the erasure plan computed by each workgroup (launches of at
+// most one workgroup per compute unit; see k_recover_tile_fused).
+template <int NL, int PQ, bool OOB>
+__global__ void __launch_bounds__((Tile<NL, PQ>::THREADS)) __attribute__((amdgpu_waves_per_eu(4)))
+    k_recover_stream_fused(uint8_t *__restrict__ data, uint32_t L, uint32_t tiles, PosArg pos, uint32_t e,
+                           const uint8_t *__restrict__ gtab, const MulTab *__restrict__ g_tw) {
+    using TT = Tile<NL, PQ>;
+    constexpr int N = TT::N;
+    __shared__ uint32_t tile[TT::WORDS];
+    __shared__ MulTab tw[kTwiddleEntries];
+    __shared__ MulTab s_tab[N];
+    __shared__ __align__(16) uint8_t s_er[N];
+    static_assert(TT::WORDS * 4 >= 1536, "the plan's scratch must fit the tile buffer");
+    stage_twiddles<TT::THREADS>(tw, g_tw);
+    uint8_t *sc = reinterpret_cast<uint8_t *>(tile);
+    recover_plan<TT::THREADS>(PlanToLds{s_tab, s_er}, N, e, pos.p, gtab,
+                              PlanScratch{sc, sc + 512, sc + 768, sc + 1024, reinterpret_cast<uint32_t *>(sc + 1280)});
+    __syncthreads();  // the plan is in s_tab / s_er, and the tile buffer is free again
+    recover_stream<NL, PQ, OOB>(data, L, tiles, s_tab, s_er, tile, tw);
+}
+
 template <int TL, bool AL>
 __global__ void __launch_bounds__(256) k_recompute_small(uint8_t *__restrict__ data, uint32_t L,
                                                          const MulTab *__restrict__ g_tw) {
@@ -331,15 +402,55 @@ hipError_t launch_recover_prep(RecPrep *out, uint32_t n, uint32_t e, const PosAr
     return hipGetLastError();
 }
 
+// Erased rows may be read at the buffer's end (see recover_rows_in) when no
+// row offset added to it wraps past 2^32.
+static bool oob_ok(int nl, uint32_t L) { return ((uint64_t)(1u << nl) + 15u) * L < (1ull << 32); }
+
+// Workgroups of `kernel` resident per compute unit (at least 1).
+template <class K>
+static uint32_t resident_groups(K kernel, int threads) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0) != hipSuccess || nb < 1) nb = 1;
+    return (uint32_t)nb;
+}
+
+// The streaming recover's grid: twice the workgroups the device holds at
+// once (measured faster than exactly one round of resident workgroups), at
+// most one per tile.
+template <int NL, int PQ>
+static uint32_t stream_grid(uint32_t tiles, uint32_t cus) {
+    static const uint32_t occ = resident_groups(k_recover_stream<NL, PQ, true>, Tile<NL, PQ>::THREADS);
+    const uint64_t g = 2ull * occ * (cus ? cus : 256u);
+    return (uint32_t)(g < tiles ? g : tiles);
+}
+
 template <int NL, bool AL>
 static hipError_t rec_one(uint8_t *data, uint32_t L, const RecPrep *prep, const MulTab *tw,
-                          hipStream_t s, int pq, int nbuf) {
+                          hipStream_t s, int pq, int nbuf, uint32_t cus) {
     const uint64_t quads = ((uint64_t)L + 3) / 4;
     if constexpr (NL <= 4) {
         hipLaunchKernelGGL((k_recover_small<NL, AL>), dim3(ceil_div(quads, 256)), dim3(256), 0, s,
                            data, L, prep, tw);
         return hipGetLastError();
     } else {
+        if constexpr (AL) {
+            if (nbuf == 1) {
+#define RS_V(PQ_)                                                                                    \
+    if (pq == PQ_) {                                                                                 \
+        const uint32_t tiles = ceil_div(quads, PQ_), grid = stream_grid<NL, PQ_>(tiles, cus);        \
+        if (oob_ok(NL, L))                                                                           \
+            hipLaunchKernelGGL((k_recover_stream<NL, PQ_, true>), dim3(grid),                        \
+                               dim3(Tile<NL, PQ_>::THREADS), 0, s, data, L, tiles, prep, tw);        \
+        else                                                                                         \
+            hipLaunchKernelGGL((k_recover_stream<NL, PQ_, false>), dim3(grid),                       \
+                               dim3(Tile<NL, PQ_>::THREADS), 0, s, data, L, tiles, prep, tw);        \
+        return hipGetLastError();                                                                    \
+    }
+                RS_V(16) RS_V(32) RS_V(64)
+#undef RS_V
+                return hipErrorInvalidValue;
+            }
+        }
 #define REC_V(PQ_, NB_)                                                                              \
     if (pq == PQ_ && nbuf == NB_) {                                                                  \
         hipLaunchKernelGGL((k_recover_tile<NL, PQ_, AL, NB_>), dim3(ceil_div(quads, PQ_)),           \
@@ -353,12 +464,12 @@ static hipError_t rec_one(uint8_t *data, uint32_t L, const RecPrep *prep, const 
 }
 
 hipError_t launch_recover(int nl, bool al, uint8_t *data, uint32_t L, const RecPrep *prep,
-                          const MulTab *tw, hipStream_t s, int pq, int nbuf) {
-    if (pq == 0) pq = default_pq(1, nl);
-    if (nbuf == 0) nbuf = default_nbuf(1, nl);
+                          const MulTab *tw, hipStream_t s, int pq, int nbuf, uint32_t cus) {
+    if (pq == 0) pq = (al || nl < 8) ? default_pq(1, nl) : 32;  // byte path: 32 quads at 256 points
+    if (nbuf == 0) nbuf = al ? default_nbuf(1, nl) : 2;
     if (!variant_ok(nl, pq, nbuf)) return hipErrorInvalidValue;
 #define RC_CASE(K) \
-    case K: return al ? rec_one<K, true>(data, L, prep, tw, s, pq, nbuf) : rec_one<K, false>(data, L, prep, tw, s, pq, nbuf);
+    case K: return al ? rec_one<K, true>(data, L, prep, tw, s, pq, nbuf, cus) : rec_one<K, false>(data, L, prep, tw, s, pq, nbuf, cus);
     switch (nl) {
         RC_CASE(1) RC_CASE(2) RC_CASE(3) RC_CASE(4) RC_CASE(5) RC_CASE(6) RC_CASE(7) RC_CASE(8)
         default: return hipErrorInvalidValue;
@@ -374,8 +485,18 @@ static hipError_t rec_fused_one(uint8_t *data, uint32_t L, const PosArg &pos, ui
         const uint32_t grid = ceil_div(quads, 256);
         if (grid > max_groups) return hipErrorNotSupported;
         hipLaunchKernelGGL((k_recover_small_fused<NL, AL>), dim3(grid), dim3(256), 0, s, data, L, pos, e, gtab, tw);
+    } else if constexpr (AL) {
+        constexpr int PQ = default_pq(1, NL);
+        const uint32_t grid = ceil_div(quads, PQ);
+        if (grid > max_groups) return hipErrorNotSupported;
+        if (oob_ok(NL, L))
+            hipLaunchKernelGGL((k_recover_stream_fused<NL, PQ, true>), dim3(grid), dim3(Tile<NL, PQ>::THREADS), 0, s,
+                               data, L, grid, pos, e, gtab, tw);
+        else
+            hipLaunchKernelGGL((k_recover_stream_fused<NL, PQ, false>), dim3(grid), dim3(Tile<NL, PQ>::THREADS), 0, s,
+                               data, L, grid, pos, e, gtab, tw);
     } else {
-        constexpr int PQ = default_pq(1, NL), NB = default_nbuf(1, NL);
+        constexpr int PQ = NL >= 8 ? 32 : 64, NB = 2;  // the byte path's default variant
         const uint32_t grid = ceil_div(quads, PQ);
         if (grid > max_groups) return hipErrorNotSupported;
         hipLaunchKernelGGL((k_recover_tile_fused<NL, PQ, AL, NB>), dim3(grid), dim3(Tile<NL, PQ>::THREADS), 0, s,

@@ -23,11 +23,13 @@ constexpr int kGtabBytes = 1024;
 
 // Default tile variant (quads per workgroup, LDS buffers) of an operation
 // (0 = encode, 1 = recover) on 2^tl points, used when the context holds no
-// tuned choice: 32 quads for the largest transforms (128-point encode with one
-// buffer, 256-point recover with two; the autotuner's picks there on an
-// MI355X), 64 quads with two buffers below.
-constexpr int default_pq(int op, int tl) { return tl >= (op == 0 ? 7 : 8) ? 32 : 64; }
-constexpr int default_nbuf(int op, int tl) { return (op == 0 && tl >= 7) ? 1 : 2; }
+// tuned choice.  Encode: 32 quads for the 128-point transform with one
+// buffer, 64 quads with two below.  Recover: 64 quads with one buffer (the
+// streaming kernel, k_recover_stream: a wave then holds one shard group, so
+// erased rows are skipped with scalar branches), measured fastest for every
+// size from 32 to 256 points; the byte path has only the two-buffer kernel.
+constexpr int default_pq(int op, int tl) { return (op == 0 && tl >= 7) ? 32 : 64; }
+constexpr int default_nbuf(int op, int tl) { return op == 1 ? 1 : (tl >= 7 ? 1 : 2); }
 
 // pq / nbuf select a tile variant for transforms of 32+ points (0 = default):
 // pq = column quads per workgroup (16, 32 or 64), nbuf = LDS exchange buffers (1 or 2).
@@ -38,8 +40,10 @@ hipError_t launch_encode_kpow2(int kl, bool aligned, const uint8_t *msg, uint8_t
                                uint32_t blocks, uint32_t msg_base, const MulTab *tw, hipStream_t s);
 hipError_t launch_recover_prep(RecPrep *out, uint32_t n, uint32_t e, const PosArg &pos,
                                const uint8_t *gtab, hipStream_t s);
+// Recover with a ready plan.  With nbuf == 1 and dword-addressable rows this
+// is the streaming kernel, whose grid is sized from `cus` compute units.
 hipError_t launch_recover(int nl, bool aligned, uint8_t *data, uint32_t L, const RecPrep *prep,
-                          const MulTab *tw, hipStream_t s, int pq = 0, int nbuf = 0);
+                          const MulTab *tw, hipStream_t s, int pq = 0, int nbuf = 0, uint32_t cus = 0);
 // Recover in one launch, each workgroup computing the plan itself (the
 // default tile variant), when the launch has at most max_groups workgroups;
 // hipErrorNotSupported (nothing launched) otherwise.

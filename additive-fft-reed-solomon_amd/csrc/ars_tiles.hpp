@@ -335,6 +335,113 @@ __device__ __forceinline__ void body_recover_tile(IO &io, const TABS &tabs, int 
 }
 
 // ---------------------------------------------------------------------------
+// Recover, one exchange buffer (the default for dword-addressable codewords).
+// The same algorithm as body_recover_tile, arranged so that every LDS word is
+// written and read once per exchange and nothing is held for the output:
+//   * the 16 erased flags of the lane's shard group are read in one LDS read
+//     (Flags16), and erased rows are neither loaded nor multiplied (u_i = 0);
+//   * Dlo(x), the derivative's low-bit terms of the phase-A output x (see
+//     body_recover_tile), is computed in registers before the exchange and
+//     added after it, so phase B's result goes back through the same buffer;
+//   * the recovered value goes into the erased shard by XOR (io.xor_out), the
+//     reference's c_p ^ g_p (rs/src/codec.rs:560-566), so the erased content
+//     g_p is never read into registers.
+// rows holds the lane's 16 input rows on entry (erased rows 0, see
+// recover_rows_in) and is read only at the start.  after_exchange() runs once
+// the tile is back in the phase-A layout, before the phase-A FFT; the
+// one-codeword kernels load the next tile's rows into `rows` there.  The caller synchronises before reusing
+// `tile` (other lanes' get_a reads of it).
+// Bit j: row q*16 + j is erased (one VGPR; the 16 flag bytes would take four).
+struct Flags16 {
+    uint32_t m;
+    __device__ __forceinline__ bool at(int j) const { return (m >> j) & 1u; }
+};
+
+// UNI: every lane of the wave holds the same shard group (64+ quads per
+// workgroup), so the mask is moved to an SGPR and the per-row tests below
+// become scalar branches that skip erased rows' loads and multiplies, and
+// surviving rows' outputs, for the whole wave.
+template <bool UNI, class TABS>
+__device__ __forceinline__ Flags16 group_flags(const TABS &tabs, int q) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(tabs.e + q * 16);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) m |= ((w[j >> 2] >> (8 * (j & 3))) & 0xffu) ? (1u << j) : 0u;
+    if constexpr (UNI) m = __builtin_amdgcn_readfirstlane(m);
+    return Flags16{m};
+}
+
+// OOB: erased rows are read at io.oob, past the end of the buffer, where a
+// buffer load returns 0 -- 16 loads without a branch between them.  The
+// offset must stay out of range after the row's scalar offset is added
+// (the launchers choose it when (n + 15) * L < 2^32); otherwise each load
+// of a surviving row sits in a branch.  With wave-uniform flags
+// (IO::kUniformGroups) the branch is scalar and skips the load outright.
+template <bool OOB, class IO>
+__device__ __forceinline__ void recover_rows_in(const IO &io, const Flags16 &f, int q, uint32_t (&x)[16]) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        if constexpr (OOB && !IO::kUniformGroups) {
+            x[j] = io.ld_in_or0((uint32_t)(q * 16), j, f.at(j));
+        } else {
+            uint32_t v = 0;
+            if (!f.at(j)) v = io.ld_in((uint32_t)(q * 16), j);
+            x[j] = v;
+        }
+    }
+}
+
+template <int NL, int PQ, class IO, class TABS, class AFTER>
+__device__ __forceinline__ void body_recover_tile1(const IO &io, const TABS &tabs, const Flags16 &f, int q, int p,
+                                                   uint32_t *tile, const MulTab *tw, const uint32_t (&rows)[16],
+                                                   AFTER &&after_exchange) {
+    using TT = Tile<NL, PQ>;
+    constexpr int Q = TT::Q, GB = TT::GB, RB = TT::RB;
+    uint32_t x[16], y[GB][Q], dlo[16];
+    // Erased rows are 0 and stay 0.  With a wave-uniform mask (PQ >= 64) their
+    // multiply is skipped; otherwise it is done (a divergent branch here costs
+    // registers).
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        if (PQ % 64 == 0 && f.at(j))
+            x[j] = 0u;
+        else
+            x[j] = gmul4(rows[j], tabs.tab(q * 16 + j));
+    }
+    run_levels<4, 0, 0, 4, true, false>(x, q * 16, tw);
+    low_derivative(x, dlo);
+    TT::put_a(tile, x, q, p);
+    __syncthreads();
+    TT::get_b(tile, y, q, p);
+#pragma unroll
+    for (int m = 0; m < GB; ++m) {
+        run_levels<RB, 4, 4, NL, true, true>(y[m], 0u, tw);
+        // Dhi in place: ascending h only reads higher, still untouched, indices
+#pragma unroll
+        for (int h = 0; h < Q; ++h) {
+            uint32_t a = 0;
+#pragma unroll
+            for (int b = 0; b < RB; ++b)
+                if (!((h >> b) & 1)) a ^= y[m][h | (1 << b)];
+            y[m][h] = a;
+        }
+        run_levels<RB, 4, 4, NL, false, true>(y[m], 0u, tw);
+    }
+    // put_b rewrites exactly the words this lane's get_b read
+    TT::put_b(tile, y, q, p);
+    __syncthreads();
+    TT::get_a(tile, x, q, p);
+    after_exchange();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) x[j] ^= dlo[j];
+    run_levels<4, 0, 0, 4, false, false>(x, q * 16, tw);
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+        if (f.at(j)) io.xor_out((uint32_t)(q * 16), j, gmul4(x[j], tabs.tab(q * 16 + j)));
+}
+
+// ---------------------------------------------------------------------------
 // Codec::recompute_data_from_parity_sharded (rs/src/codec.rs:321-340):
 // message = FFT_T(IFFT_T(parity, 0), omega_T), n = 2T.  Input rows: parity
 // shards 0..T-1; output rows: the same buffer, shards T..2T-1.
@@ -504,10 +611,11 @@ __device__ __forceinline__ void recover_prep_body(RecPrep *__restrict__ out, uin
 // I/O policies and multiplier sources.
 
 // One codeword per launch: buffer resources, scalar per-shard offsets.
-template <bool AL>
+template <bool AL, bool UNI = false>
 struct BufIO {
+    static constexpr bool kUniformGroups = UNI;  // every lane of a wave holds one shard group
     Rows in, out;
-    uint32_t L, off, nb;
+    uint32_t L, off, nb, oob;  // oob: the input buffer's size (see ld_in_or0)
     __device__ __forceinline__ uint32_t ld_in(uint32_t g, int j) const {
         return ldq<AL>(in, off + g * L, (uint32_t)j * L, nb);
     }
@@ -517,15 +625,26 @@ struct BufIO {
     __device__ __forceinline__ void st_out(uint32_t g, int j, uint32_t v) const {
         stq<AL>(out, off + g * L, (uint32_t)j * L, v, nb);
     }
+    // input row g + j, or 0 if `zero` (read at `oob`, past the buffer's end)
+    __device__ __forceinline__ uint32_t ld_in_or0(uint32_t g, int j, bool zero) const {
+        static_assert(AL, "ld_in_or0 needs dword-addressable rows");
+        return __builtin_amdgcn_raw_buffer_load_b32(in.r, (int)(zero ? oob : off + g * L), (int)((uint32_t)j * L), 0);
+    }
+    // out ^= v (dword rows only)
+    __device__ __forceinline__ void xor_out(uint32_t g, int j, uint32_t v) const {
+        static_assert(AL, "xor_out needs dword-addressable rows");
+        if (nb) ars_buffer_atomic_xor((int)v, out.r, (int)(off + g * L), (int)((uint32_t)j * L), 0);
+    }
 };
 
-template <bool AL>
-__device__ __forceinline__ BufIO<AL> make_bufio(const void *in, uint32_t in_bytes, void *out,
-                                                uint32_t out_bytes, uint32_t quad, uint32_t L) {
-    BufIO<AL> io;
+template <bool AL, bool UNI = false>
+__device__ __forceinline__ BufIO<AL, UNI> make_bufio(const void *in, uint32_t in_bytes, void *out,
+                                                     uint32_t out_bytes, uint32_t quad, uint32_t L) {
+    BufIO<AL, UNI> io;
     io.in = make_rows(in, in_bytes);
     io.out = make_rows(out, out_bytes);
     io.L = L;
+    io.oob = in_bytes;
     const uint32_t col = quad * 4u;
     io.nb = col < L ? ((L - col) < 4u ? (L - col) : 4u) : 0u;
     io.off = (AL && io.nb == 0u) ? 0u : col;

@@ -91,7 +91,7 @@ int ars_autotune(ars_ctx *ctx, uint32_t n, uint32_t k, size_t L, ars_tune_result
                 else
                     us = time_us([&] { return ars::launch_recover(tl, true, buf, (uint32_t)L,
                                                                   reinterpret_cast<ars::RecPrep *>(plan), ctx->d_tw, s,
-                                                                  v.pq, v.nbuf); },
+                                                                  v.pq, v.nbuf, (uint32_t)ctx->num_cus); },
                                  s, a, b);
                 (void)hipGetLastError();
                 if (us < 0) continue;

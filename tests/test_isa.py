@@ -79,18 +79,25 @@ def test_tile_kernels_address_lds_directly(asm):
 
 def test_default_kernels_do_not_spill(asm):
     # the variants the launchers pick without tuning: PQ 32 / NBUF 1 for the
-    # 128-point encode, PQ 32 / NBUF 2 for the 256-point recover, PQ 64 / NBUF 2
-    # below; the one-launch recovers use the same variants
+    # 128-point encode, PQ 64 / NBUF 2 below; recover: the streaming kernel with
+    # 64 quads (either load form), and on the byte path PQ 32 / NBUF 2 at 256
+    # points, PQ 64 / NBUF 2 below; the one-launch recovers use the same variants.
+    # A spill matters doubly in the streaming kernel: a scratch reload waits for
+    # every memory operation issued before it, the prefetched rows included.
     want = [r"k_encode_tileILi7ELi32ELb1ELb0ELi1ELb1E", r"k_encode_tileILi6ELi64ELb1ELb0ELi2E",
-            r"k_recover_tileILi8ELi32ELb1ELi2E", r"k_recover_tileILi7ELi64ELb1ELi2E",
-            r"k_recover_tile_fusedILi8ELi32ELb1ELi2E", r"k_recover_tile_fusedILi5ELi64ELb0ELi2E",
+            r"k_recover_streamILi8ELi64ELb1E", r"k_recover_streamILi8ELi64ELb0E",
+            r"k_recover_streamILi7ELi64ELb1E", r"k_recover_streamILi5ELi64ELb1E",
+            r"k_recover_stream_fusedILi8ELi64ELb1E", r"k_recover_tileILi8ELi32ELb0ELi2E",
+            r"k_recover_tile_fusedILi8ELi32ELb0ELi2E", r"k_recover_tile_fusedILi5ELi64ELb0ELi2E",
             r"k_recover_small_fusedILi4ELb0E", r"k_syndrome_tile", r"k_decode_columns"]
     seen = {}
-    for src, fns in asm.items():
-        for name, body in fns.items():
-            for w in want:
-                if re.search(w, name):
-                    m = re.search(r"; ScratchSize: (\d+)", "\n".join(body))
-                    seen[name[:90]] = int(m.group(1)) if m else 0
-    assert len(seen) >= len(want), seen
-    assert all(v == 0 for v in seen.values()), seen
+    unmatched = []
+    for w in want:
+        hits = {name[:90]: body for fns in asm.values() for name, body in fns.items() if re.search(w, name)}
+        if not hits:
+            unmatched.append(w)
+        for name, body in hits.items():
+            m = re.search(r"; ScratchSize: (\d+)", "\n".join(body))
+            seen[name] = int(m.group(1)) if m else 0
+    assert not unmatched, f"no kernel matches {unmatched}"
+    assert all(v == 0 for v in seen.values()), {k: v for k, v in seen.items() if v}
