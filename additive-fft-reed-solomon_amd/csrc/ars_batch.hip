@@ -28,7 +28,6 @@ namespace ars {
 // Packed items: each lane addresses its own codeword as a 32-bit offset
 // from the item's base pointers, through one scalar buffer resource.
 struct PackIO {
-    static constexpr bool kUniformGroups = false;
     Rows in, out;
     uint32_t vin, vout, L, nb;
     __device__ __forceinline__ uint32_t ld_in(uint32_t g, int j) const {
@@ -39,9 +38,6 @@ struct PackIO {
     }
     __device__ __forceinline__ void st_out(uint32_t g, int j, uint32_t v) const {
         if (nb) __builtin_amdgcn_raw_buffer_store_b32(v, out.r, (int)(vout + (g + (uint32_t)j) * L), 0, 0);
-    }
-    __device__ __forceinline__ void xor_out(uint32_t g, int j, uint32_t v) const {
-        if (nb) ars_buffer_atomic_xor((int)v, out.r, (int)(vout + (g + (uint32_t)j) * L), 0, 0);
     }
 };
 
@@ -159,11 +155,13 @@ __device__ __forceinline__ void run_body(const BItem &it, IO &io, int q, int p, 
     } else if constexpr (V == 2) {
         if constexpr (TL <= 7) body_encode_kpow2_tile<TL, PQ, 1>(io, q, p, tile, it.arg & 0xffffu, it.arg >> 16, tw);
     } else {
+        // The one-buffer streaming body (body_recover_tile1) measured slower
+        // here: 32-quad items leave the erased flags lane-varying, and this
+        // kernel's shared 128-VGPR budget spills, so its scratch reloads wait
+        // for the atomic outputs (mixed batch 1.94 vs 2.24 TB/s, one-shape
+        // (256,128) 64 KiB recover 1.53 vs 1.69 TB/s).
         LdsTabs tabs{s_tab + si * (1u << TL), s_er + si * (1u << TL)};
-        const Flags16 f = group_flags<false>(tabs, q);
-        uint32_t x[16];
-        recover_rows_in<false>(io, f, q, x);
-        body_recover_tile1<TL, PQ>(io, tabs, f, q, p, tile, tw, x, [] {});
+        body_recover_tile<TL, PQ, 1>(io, tabs, q, p, tile, tw);
     }
 }
 
@@ -255,7 +253,7 @@ __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per
     __shared__ uint32_t tile[8192];  // 2^TL shards x batch_lanes(TL) quads, every TL >= 5
     __shared__ MulTab tw[kTwiddleEntries];
     __shared__ MulTab s_tab[kBatchPlanSlots];
-    __shared__ __align__(16) uint8_t s_er[kBatchPlanSlots];
+    __shared__ uint8_t s_er[kBatchPlanSlots];
     __shared__ uint32_t s_item;
     BHeader *h = reinterpret_cast<BHeader *>(arena);
     const BItem *items = reinterpret_cast<const BItem *>(arena + h->off_items);
