@@ -53,6 +53,7 @@ SIGNATURES = {
     "ars_version": (ctypes.c_char_p, []),
     "ars_encode": (_i, [_vp, _u32, _u32, _vp, _vp, _sz, _u32, _vp]),
     "ars_recover": (_i, [_vp, _u32, _u32, _vp, _sz, _vp, _u32, _vp]),
+    "ars_recover_noalloc": (_i, [_vp, _u32, _u32, _vp, _sz, _vp, _u32, _vp]),
     "ars_recompute_data": (_i, [_vp, _u32, _u32, _vp, _sz, _vp]),
     "ars_decode": (_i, [_vp, _u32, _u32, _vp, _sz, _u32, _vp, ctypes.POINTER(ctypes.c_uint64), _vp]),
     "ars_recover_plan_size": (_sz, []),
@@ -191,6 +192,14 @@ class Context:
         return _check(lib().ars_recover(self.handle, n, k, d_received, shard_len,
                                         pos.ctypes.data if len(pos) else None, len(pos),
                                         _stream_ptr(stream)))
+
+    def recover_noalloc(self, n, k, d_received, shard_len, positions, stream=None):
+        """ars_recover_noalloc: ars_recover with the plan in the library's
+        per-GPU scratch array (no allocation; calls on one GPU must not overlap)."""
+        pos = np.ascontiguousarray(np.asarray(positions, dtype=np.uint8))
+        return _check(lib().ars_recover_noalloc(self.handle, n, k, d_received, shard_len,
+                                                pos.ctypes.data if len(pos) else None, len(pos),
+                                                _stream_ptr(stream)))
 
     def recover_plan(self, n, k, positions, d_plan, stream=None):
         pos = np.ascontiguousarray(np.asarray(positions, dtype=np.uint8))

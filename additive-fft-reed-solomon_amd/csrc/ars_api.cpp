@@ -64,6 +64,14 @@ namespace {
 using ars::check_code;
 bool aligned4(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; }
 
+// Scratch of ars_recover_noalloc: room for one erasure plan (n <= 256).  Every
+// GPU the library runs on holds its own copy until the process ends.  Host
+// code must not use this name as a device address: each context takes its
+// device's address with hipGetSymbolAddress in ars_create and passes that.
+constexpr size_t kNoAllocScratchBytes = 6656;
+static_assert(sizeof(ars::RecPrep) <= kNoAllocScratchBytes, "one plan must fit the scratch");
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
 // The plan buffer ars_recover keeps for `s`, created on the stream's first
 // call.  Work on one stream runs in issue order, so a call's plan kernel
 // overwrites the buffer only after the previous call's tile kernel has read
@@ -123,6 +131,13 @@ int ars_create(int device, ars_ctx **out) {
         hipFree(c->d_gtab);
         delete c;
         return hip_fail(e, "context tables");
+    }
+    // this device's copy of the no-allocation scratch (the current device is `device`)
+    if ((e = hipGetSymbolAddress(reinterpret_cast<void **>(&c->noalloc_plan), HIP_SYMBOL(scratch))) != hipSuccess) {
+        hipFree(c->d_tw);
+        hipFree(c->d_gtab);
+        delete c;
+        return hip_fail(e, "hipGetSymbolAddress(scratch)");
     }
     *out = c;
     return ARS_OK;
@@ -255,6 +270,33 @@ int ars_recover(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_
     hipError_t ef = hipFreeAsync(prep, s);
     if (e != hipSuccess) return hip_fail(e, "recover launch");
     if (ef != hipSuccess) return hip_fail(ef, "hipFreeAsync");
+    return ARS_TRUE;
+}
+
+int ars_recover_noalloc(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_t L,
+                        const uint8_t *positions, uint32_t e_cnt, void *stream) {
+    if (!ctx || !d_received) return fail(ARS_E_NULL, "null context or buffer");
+    int rc = check_code(n, k, L);  // n <= 256: the plan fits the scratch (static_assert above)
+    if (rc) return rc;
+    const uint32_t T = n - k;
+    if (e_cnt > T) return ARS_FALSE;
+    if (e_cnt == 0) return ARS_TRUE;
+    ars::PosArg pa;
+    if ((rc = validate_positions(n, positions, e_cnt, pa))) return rc;
+    if (!ctx->noalloc_plan) return fail(ARS_E_STATE, "context has no scratch address");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const bool al = (L % 4 == 0) && aligned4(d_received);
+    const int nl = ilog2(n);
+    e = ars::launch_recover_fused(nl, al, d_received, L, pa, e_cnt, ctx->d_gtab, ctx->d_tw, s,
+                                  (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256));
+    if (e == hipSuccess) return ARS_TRUE;
+    if (e != hipErrorNotSupported) return hip_fail(e, "recover launch");
+    if ((e = ars::launch_recover_prep(ctx->noalloc_plan, n, e_cnt, pa, ctx->d_gtab, s)) != hipSuccess ||
+        (e = ars::launch_recover(nl, al, d_received, L, ctx->noalloc_plan, ctx->d_tw, s, ctx->tune_pq[1][nl],
+                                 ctx->tune_nbuf[1][nl], (uint32_t)ctx->num_cus)) != hipSuccess)
+        return hip_fail(e, "recover launch");
     return ARS_TRUE;
 }
 

@@ -36,6 +36,9 @@ struct ars_ctx {
     // (at most kMaxPlanSlots; later streams allocate per call)
     std::mutex plan_mu;
     std::vector<std::unique_ptr<ars::PlanSlot>> plan_slots;
+    // this device's address of the file-scope plan scratch of
+    // ars_recover_noalloc (ars_api.cpp), from hipGetSymbolAddress at creation
+    ars::RecPrep *noalloc_plan = nullptr;
 };
 
 namespace ars {

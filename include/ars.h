@@ -107,6 +107,21 @@ int ars_recover_plan(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *positi
 int ars_recover_with_plan(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received,
                           size_t shard_len, const void *d_plan, void *stream);
 
+/* ars_recover without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  Same arguments,
+ * semantics and return codes as ars_recover; it calls no hipMalloc, hipFree or
+ * allocating library routine.  Launches of more than one workgroup per
+ * compute unit put the erasure plan in a scratch array of the library (one
+ * plan, 6.5 KiB, for n <= 256; larger n is rejected with ARS_E_INVAL by the
+ * (n, k) check) that each GPU holds until the process ends.  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other host threads, other contexts on that device) race, so serialize them
+ * on one stream.  Each call's plan kernel rewrites every entry its recover
+ * kernel reads, so nothing carries over from the previous call. */
+int ars_recover_noalloc(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_t shard_len,
+                        const uint8_t *positions, uint32_t e, void *stream);
+
 /* Codec::recompute_data_from_parity_sharded (rs/src/codec.rs:321-340):
  * n = 2T only; rewrites the message shards T..n-1 from the parity shards. */
 int ars_recompute_data(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received,
