@@ -340,7 +340,8 @@ class ServerStats(ctypes.Structure):
                 ("algorithmic_bytes", ctypes.c_uint64), ("p50_latency_s", ctypes.c_double),
                 ("p99_latency_s", ctypes.c_double), ("max_latency_s", ctypes.c_double),
                 ("mean_batch_jobs", ctypes.c_double), ("first_arrival_s", ctypes.c_double),
-                ("last_completion_s", ctypes.c_double)]
+                ("last_completion_s", ctypes.c_double), ("batched_jobs", ctypes.c_uint64),
+                ("direct_jobs", ctypes.c_uint64)]
 
 
 class Server:

@@ -28,6 +28,7 @@
 
 #include "../../include/ars.h"
 #include "ars_ctx.hpp"
+#include "ars_serve_poll.hpp"
 
 namespace {
 
@@ -101,6 +102,7 @@ struct Slot {
     std::vector<ars_job> jobs;    // run in the batched kernel
     std::vector<ars_job> direct;  // run as plain calls (codewords of direct_min_bytes or more)
     bool busy = false;
+    bool ran_batch = false;  // the last launch used the batched kernel
 };
 
 }  // namespace
@@ -123,7 +125,7 @@ struct ars_server {
     Clock::time_point t0;
     // statistics since the last reset
     std::vector<double> lat;
-    uint64_t submitted = 0, completed = 0, rejected = 0, batches = 0, batch_jobs = 0, alg = 0;
+    uint64_t submitted = 0, completed = 0, rejected = 0, batches = 0, batch_jobs = 0, direct_jobs = 0, alg = 0;
     double first_arrival = -1, last_completion = 0;
 };
 
@@ -179,13 +181,15 @@ void launch_slot(ars_server *s, Slot &sl, bool loaded) {
     // off only over several jobs: fewer run as plain calls.  With no other
     // slot busy the GPU would otherwise idle through that fixed cost, so the
     // bar is higher; under load, batching keeps the launches few.
-    const size_t min_jobs = s->cfg.min_batch_jobs ? s->cfg.min_batch_jobs : (loaded ? kMinBatchJobs : kIdleBatchJobs);
+    const size_t min_jobs =
+        (size_t)ars::min_batch_jobs(s->cfg.min_batch_jobs, loaded, kMinBatchJobs, kIdleBatchJobs, s->cfg.max_batch_jobs);
     if (!sl.jobs.empty() && sl.jobs.size() < min_jobs) {
         in_direct.insert(in_direct.end(), in_batch.begin(), in_batch.end());
         sl.direct.insert(sl.direct.end(), sl.jobs.begin(), sl.jobs.end());
         in_batch.clear();
         sl.jobs.clear();
     }
+    sl.ran_batch = !sl.jobs.empty();
     const int rc_b = sl.jobs.empty()
                          ? ARS_OK
                          : ars_batch_run(sl.batch, sl.jobs.data(), (uint32_t)sl.jobs.size(), 0, sl.stream);
@@ -232,12 +236,16 @@ void dispatcher(ars_server *s) {
     // request arriving shortly after the last one would otherwise pay
     const auto linger = std::chrono::microseconds(500);
     Clock::time_point last_work = Clock::now();
+    std::vector<ars::SlotPoll> poll(s->slots.size(), ars::SlotPoll::kRunning);
     for (;;) {
         // 1. poll the slots' events without the lock (the slots belong to
         //    this thread), so submitters are not held up meanwhile
         bool any_done = false, busy = false;
         for (size_t i = 0; i < s->slots.size(); ++i) {
-            done[i] = s->slots[i].busy && hipEventQuery(s->slots[i].ev) == hipSuccess;
+            poll[i] = s->slots[i].busy ? ars::classify_slot_poll((int)hipEventQuery(s->slots[i].ev), (int)hipSuccess,
+                                                                 (int)hipErrorNotReady)
+                                       : ars::SlotPoll::kRunning;
+            done[i] = poll[i] != ars::SlotPoll::kRunning;
             any_done |= done[i] != 0;
             busy |= s->slots[i].busy;
         }
@@ -262,6 +270,7 @@ void dispatcher(ars_server *s) {
             if (!done[i]) continue;
             Slot &sl = s->slots[i];
             for (Req *q : sl.reqs) {
+                q->status = ars::retired_status(poll[i], q->status, ARS_E_HIP);
                 s->pool->release(q->off, q->bytes);
                 finish(s, q, now);
             }
@@ -288,13 +297,17 @@ void dispatcher(ars_server *s) {
             const size_t take = std::min<size_t>(s->admitted.size(), s->cfg.max_batch_jobs);
             sl.reqs.assign(s->admitted.begin(), s->admitted.begin() + take);
             s->admitted.erase(s->admitted.begin(), s->admitted.begin() + take);
-            s->batches++;
-            s->batch_jobs += take;
             bool loaded = false;
             for (const Slot &o : s->slots) loaded |= o.busy;
             lk.unlock();
             launch_slot(s, sl, loaded);
             lk.lock();
+            // only slots that ran the batched kernel count as batches
+            if (sl.ran_batch) {
+                s->batches++;
+                s->batch_jobs += sl.jobs.size();
+            }
+            s->direct_jobs += sl.direct.size();
         }
         bool any_busy = false;
         for (const Slot &sl : s->slots) any_busy |= sl.busy;
@@ -444,6 +457,8 @@ int ars_server_stats_get(ars_server *s, ars_server_stats *o) {
     o->batches = s->batches;
     o->algorithmic_bytes = s->alg;
     o->mean_batch_jobs = s->batches ? (double)s->batch_jobs / (double)s->batches : 0.0;
+    o->batched_jobs = s->batch_jobs;
+    o->direct_jobs = s->direct_jobs;
     o->first_arrival_s = s->first_arrival;
     o->last_completion_s = s->last_completion;
     if (!s->lat.empty()) {
@@ -464,7 +479,7 @@ int ars_server_stats_reset(ars_server *s) {
     if (!s) return ars::set_error(ARS_E_NULL, "null server");
     std::lock_guard<std::mutex> lk(s->mu);
     s->lat.clear();
-    s->submitted = s->completed = s->rejected = s->batches = s->batch_jobs = s->alg = 0;
+    s->submitted = s->completed = s->rejected = s->batches = s->batch_jobs = s->direct_jobs = s->alg = 0;
     s->first_arrival = -1;
     s->last_completion = 0;
     return ARS_OK;
@@ -537,7 +552,9 @@ int ars_plain_replay(ars_ctx *ctx, const ars_request *reqs, const double *arriva
         hipSetDevice(ctx->device);
         for (uint32_t i = 0; i < count; ++i) {
             while (issued.load(std::memory_order_acquire) <= i) std::this_thread::yield();
-            (void)hipEventSynchronize(ring[i % R]);
+            // a failed kernel or copy on the stream fails the request (st[i]
+            // is not written by the issuing thread after `issued` passes i)
+            if (hipEventSynchronize(ring[i % R]) != hipSuccess && st[i] >= 0) st[i] = ARS_E_HIP;
             lat[i] = std::chrono::duration<double>(Clock::now() - due(i)).count();
             retired.store(i + 1, std::memory_order_release);
         }

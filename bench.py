@@ -186,11 +186,13 @@ def valu_roofline(kernel, seconds, rates):
             "peak_from": "live v_xor_b32 and v_perm_b32 issue rates weighted by the kernel's v_perm share"}
 
 
-def bench_batched(ars, torch, np, stream, count=4096, seed=44, reps=3):
-    """Mixed sizes, one persistent launch vs one launch per codeword: `count`
-    codewords with n ~ U{32,64,128,256}, k = n/2, shard_len log-uniform in
-    [64 B, 64 KiB] (multiple of 64); each is encoded, then recovers
-    e ~ U{0..T} random erasures."""
+def c3_workload(np, count=4096, seed=44):
+    """The mixed batch (C3): `count` codewords with n ~ U{32,64,128,256},
+    k = n/2, shard_len log-uniform in [64 B, 64 KiB] (multiple of 64), laid
+    out back to back (256-byte aligned) in one pool; each is encoded, then
+    recovers e ~ U{0..T} random erasures.  Returns the shapes, pool offsets and
+    erasure positions; the pool's bytes come from a torch generator seeded
+    with `seed`."""
     rng = np.random.default_rng(seed)
     ns = rng.choice([32, 64, 128, 256], count)
     Ls = (64 * np.maximum(1, np.round(2.0 ** rng.uniform(6, 16, count) / 64))).astype(np.int64)
@@ -198,24 +200,44 @@ def bench_batched(ars, torch, np, stream, count=4096, seed=44, reps=3):
     offs = np.zeros(count, np.int64)
     offs[1:] = np.cumsum((sizes + 255) // 256 * 256)[:-1]
     total = int(offs[-1] + sizes[-1])
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(seed)
-    pool = torch.randint(0, 256, (total,), dtype=torch.uint8, device="cuda", generator=gen)
+    positions = []
+    for i in range(count):
+        T = int(ns[i]) - int(ns[i]) // 2
+        e = int(rng.integers(0, T + 1))
+        positions.append(rng.permutation(int(ns[i]))[:e])
+    return {"ns": ns, "Ls": Ls, "offs": offs, "total": total, "positions": positions, "seed": seed}
+
+
+def c3_jobs(ars, w, pool):
+    """ars.Batch job arrays (encode, then recover) of a c3_workload over `pool`."""
     enc, rec = [], []
     enc_bytes = rec_bytes = 0
-    for i in range(count):
-        n, L, o = int(ns[i]), int(Ls[i]), int(offs[i])
+    for i in range(len(w["ns"])):
+        n, L, o = int(w["ns"][i]), int(w["Ls"][i]), int(w["offs"][i])
         k = n // 2
         T = n - k
         enc.append({"op": "encode", "n": n, "k": k, "shard_len": L,
                     "message": pool[o + T * L:o + n * L], "parity": pool[o:o + T * L]})
-        e = int(rng.integers(0, T + 1))
+        e = len(w["positions"][i])
         rec.append({"op": "recover", "n": n, "k": k, "shard_len": L, "codeword": pool[o:o + n * L],
-                    "positions": rng.permutation(n)[:e]})
+                    "positions": w["positions"][i]})
         enc_bytes += n * L
         rec_bytes += (n + e) * L if e else 0
     earr, ekeep = ars.Batch.make_jobs(enc)
     rarr, rkeep = ars.Batch.make_jobs(rec)
+    return earr, rarr, (ekeep, rkeep), enc_bytes + rec_bytes
+
+
+def bench_batched(ars, torch, np, stream, count=4096, seed=44, reps=3):
+    """Mixed sizes (c3_workload), one persistent launch vs one launch per
+    codeword.  Before timing, one batched pass and one plain pass from the
+    same starting bytes must leave identical pools."""
+    w = c3_workload(np, count, seed)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    pool = torch.randint(0, 256, (w["total"],), dtype=torch.uint8, device="cuda", generator=gen)
+    earr, rarr, keep, byts = c3_jobs(ars, w, pool)
+    sizes = w["ns"] * w["Ls"]
     b = ars.Batch(count)
     ctx = b.ctx
 
@@ -240,9 +262,21 @@ def bench_batched(ars, torch, np, stream, count=4096, seed=44, reps=3):
         torch.cuda.synchronize()
         return a.elapsed_time(z) / r * 1e-3, (time.perf_counter() - w0) / r
 
+    # (the copies run on torch's current stream, the jobs on `stream`)
+    start = pool.clone()
+    torch.cuda.synchronize()
+    batched()
+    torch.cuda.synchronize()
+    got = pool.clone()
+    pool.copy_(start)
+    torch.cuda.synchronize()
+    plain()
+    torch.cuda.synchronize()
+    if not torch.equal(got, pool):
+        raise RuntimeError("batched run differs from plain calls on the mixed workload")
+    del got, start
     s_b, w_b = timed(batched, reps)
     s_p, w_p = timed(plain, reps)
-    byts = enc_bytes + rec_bytes
     info = b.info()
     b.close()
     del pool
@@ -256,39 +290,88 @@ def bench_batched(ars, torch, np, stream, count=4096, seed=44, reps=3):
         "plain_ms": round(max(s_p, w_p) * 1e3, 3),
         "speedup": round(max(s_p, w_p) / max(s_b, w_b), 2),
         "items_last_run": info["items"], "grid": info["grid"],
+        "checked": "one batched pass equals one plain-call pass, whole pool compared",
     }
 
 
-def bench_served(ars, torch, np, rates=(1000, 2000, 4000, 8000, 12000), per_rate=1500, seed=45):
-    """Poisson arrivals, 70 % encode / 30 % recover, (n,k) ~ U{(64,32),(128,96),
-    (256,128),(256,224)}, shard_len log-uniform 1 KiB-4 MiB (multiple of 64),
-    recover e ~ U{1..T}; 64 GiB HBM admission budget; payloads are
-    device-resident (the pool slot's bytes), so latency is queueing + GPU.
-    Each trace is then replayed through plain calls (ars_plain_replay: one
-    call per request, in arrival order on one stream, no batching) for the
-    plain-call latency and throughput beside the server's."""
-    shapes = [(64, 32), (128, 96), (256, 128), (256, 224)]
+C4_SHAPES = [(64, 32), (128, 96), (256, 128), (256, 224)]
+
+
+def c4_trace(np, rate, count, seed):
+    """The served stream (C4): Poisson arrivals at `rate` requests/s, 70 %
+    encode / 30 % recover, (n,k) ~ U{(64,32),(128,96),(256,128),(256,224)},
+    shard_len log-uniform 1 KiB-4 MiB (multiple of 64), recover e ~ U{1..T}.
+    Returns [(op, n, k, L, positions)] and the arrival times (s)."""
+    rng = np.random.default_rng(seed)
+    trace = []
+    for i in range(count):
+        n, k = C4_SHAPES[rng.integers(4)]
+        L = int(64 * max(1, round(2.0 ** rng.uniform(10, 22) / 64)))
+        if rng.random() < 0.7:
+            trace.append(("encode", n, k, L, None))
+        else:
+            e = int(rng.integers(1, n - k + 1))
+            trace.append(("recover", n, k, L, rng.permutation(n)[:e]))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    return trace, arrivals
+
+
+def check_served(ars, torch, np, srv, ctx, count=32, seed=4501):
+    """A short C4 trace with host payloads through the server: every output
+    must equal a single ars_encode / ars_recover call on the same input.
+    Returns the number of requests checked."""
+    trace, arrivals = c4_trace(np, 4000, count, seed)
+    reqs = (ars.Request * count)()
+    keep, outs, expect = [], [], []
+    for i, (op, n, k, L, pos) in enumerate(trace):
+        T = n - k
+        msg = torch.randint(0, 256, (k * L,), dtype=torch.uint8, device="cuda")
+        if op == "encode":
+            exp = torch.zeros(T * L, dtype=torch.uint8, device="cuda")
+            ctx.encode(n, k, msg.data_ptr(), exp.data_ptr(), L)
+            inp = msg.cpu().numpy()
+        else:
+            cw = torch.zeros(n * L, dtype=torch.uint8, device="cuda")
+            ctx.encode(n, k, msg.data_ptr(), cw.data_ptr(), L)
+            cw[T * L:] = msg
+            cw.view(n, L)[torch.from_numpy(pos.astype(np.int64)).cuda()] = 0
+            inp = cw.cpu().numpy()
+            exp = cw
+            ctx.recover(n, k, exp.data_ptr(), L, pos)
+        torch.cuda.synchronize()
+        expect.append(exp.cpu().numpy())
+        outs.append(np.zeros_like(expect[-1]))
+        reqs[i] = ars.Server.request(op, n, k, L, positions=pos if pos is not None else (), host_in=inp,
+                                     host_out=outs[-1], keep=keep)
+        keep.append(inp)
+    st, _ = srv.replay(reqs, arrivals)
+    for i, (op, n, k, L, pos) in enumerate(trace):
+        if st[i] != (0 if op == "encode" else 1) or not np.array_equal(outs[i], expect[i]):
+            raise RuntimeError(f"served request {i} ({op} n={n} k={k} L={L}) differs from a single call")
+    return count
+
+
+def bench_served(ars, torch, np, rates=(2000, 8000, 16000, 24000, 32000, 48000), per_rate=1500, seed=45):
+    """Poisson arrivals (c4_trace) under a 64 GiB HBM admission budget;
+    payloads are device-resident (the pool slot's bytes), so latency is
+    queueing + GPU.  Each trace is then replayed through plain calls
+    (ars_plain_replay: one call per request, in arrival order on one stream,
+    no batching) for the plain-call latency and throughput beside the
+    server's."""
     srv = ars.Server(hbm_budget=64 << 30, max_batch_jobs=1024, pipeline_depth=2)
     ctx = ars.Context(0)
+    checked = check_served(ars, torch, np, srv, ctx)
     out = []
     for ri, rate in enumerate(rates):
-        rng = np.random.default_rng(seed + ri)
+        trace, arrivals = c4_trace(np, rate, per_rate, seed + ri)
         reqs = (ars.Request * per_rate)()
         keep = []
         alg = 0
         biggest = 0
-        for i in range(per_rate):
-            n, k = shapes[rng.integers(4)]
-            L = int(64 * max(1, round(2.0 ** rng.uniform(10, 22) / 64)))
+        for i, (op, n, k, L, pos) in enumerate(trace):
             biggest = max(biggest, n * L)
-            if rng.random() < 0.7:
-                reqs[i] = ars.Server.request("encode", n, k, L)
-                alg += n * L
-            else:
-                e = int(rng.integers(1, n - k + 1))
-                reqs[i] = ars.Server.request("recover", n, k, L, positions=rng.permutation(n)[:e], keep=keep)
-                alg += (n + e) * L
-        arrivals = np.cumsum(rng.exponential(1.0 / rate, per_rate))
+            reqs[i] = ars.Server.request(op, n, k, L, positions=pos, keep=keep)
+            alg += n * L if op == "encode" else (n + len(pos)) * L
         srv.reset_stats()
         st, lat = srv.replay(reqs, arrivals)
         stt = srv.stats()
@@ -305,6 +388,7 @@ def bench_served(ars, torch, np, rates=(1000, 2000, 4000, 8000, 12000), per_rate
                     "throughput_GBps": round(stt["algorithmic_bytes"] / span / 1e9, 1),
                     "achieved_req_per_s": round(per_rate / span, 1),
                     "mean_batch_jobs": round(stt["mean_batch_jobs"], 2),
+                    "batched_jobs": int(stt["batched_jobs"]), "direct_jobs": int(stt["direct_jobs"]),
                     "plain": {"ok": int((pst >= 0).sum()),
                               "p50_ms": round(float(np.percentile(plat, 50)) * 1e3, 3),
                               "p99_ms": round(float(np.percentile(plat, 99)) * 1e3, 3),
@@ -317,7 +401,20 @@ def bench_served(ars, torch, np, rates=(1000, 2000, 4000, 8000, 12000), per_rate
                         "(256,128),(256,224)}, L log-uniform 1 KiB-4 MiB, e~U{1..T}, 64 GiB budget",
             "plain": "the same trace, one ars_encode / ars_recover call per request in arrival order on "
                      "one stream (ars_plain_replay); latency = completion - arrival",
+            "checked_requests": checked,
+            "sustained_req_per_s": sustained(out),
             "sweep": out}
+
+
+def sustained(sweep, keep_up=0.95):
+    """Highest offered rate each path kept up with (achieved >= 95 % of offered)."""
+    best = {"server": 0, "plain": 0}
+    for r in sweep:
+        if r["achieved_req_per_s"] >= keep_up * r["rate_req_per_s"]:
+            best["server"] = max(best["server"], r["rate_req_per_s"])
+        if r["plain"]["achieved_req_per_s"] >= keep_up * r["rate_req_per_s"]:
+            best["plain"] = max(best["plain"], r["rate_req_per_s"])
+    return best
 
 
 def bench_served_small(ars, torch, np, rates=(20000, 50000, 100000, 200000), per_rate=4000, seed=48):
@@ -361,6 +458,7 @@ def bench_served_small(ars, torch, np, rates=(20000, 50000, 100000, 200000), per
                     "throughput_GBps": round(stt["algorithmic_bytes"] / span / 1e9, 1),
                     "achieved_req_per_s": round(per_rate / span, 1),
                     "mean_batch_jobs": round(stt["mean_batch_jobs"], 2),
+                    "batched_jobs": int(stt["batched_jobs"]), "direct_jobs": int(stt["direct_jobs"]),
                     "plain": {"ok": int((pst >= 0).sum()),
                               "p50_ms": round(float(np.percentile(plat, 50)) * 1e3, 3),
                               "p99_ms": round(float(np.percentile(plat, 99)) * 1e3, 3),
@@ -371,6 +469,7 @@ def bench_served_small(ars, torch, np, rates=(20000, 50000, 100000, 200000), per
     ctx.close()
     return {"workload": "Poisson arrivals, 70% encode / 30% recover, n in {32,64,128,256}, k=n/2, "
                         "L log-uniform 64 B-64 KiB, e~U{1..T}; server vs the same trace through plain calls",
+            "sustained_req_per_s": sustained(out),
             "sweep": out}
 
 

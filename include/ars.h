@@ -231,7 +231,8 @@ typedef struct ars_server_config {
     uint32_t min_batch_jobs;   /* a batch with fewer jobs left for the kernel runs them as
                                   plain calls too, its fixed cost exceeding theirs (0 = 4
                                   while another batch is in flight, 16 on an otherwise idle
-                                  GPU; 1 = always use the kernel) */
+                                  GPU; 1 = always use the kernel); never more than
+                                  max_batch_jobs */
     uint32_t reserved;         /* 0 */
 } ars_server_config;
 
@@ -250,10 +251,14 @@ typedef struct ars_request {
 } ars_request;
 
 typedef struct ars_server_stats {
-    uint64_t submitted, completed, rejected, batches;
+    uint64_t submitted, completed, rejected;
+    uint64_t batches;                /* slot launches that ran the batched kernel */
     uint64_t algorithmic_bytes;      /* over completed requests */
-    double p50_latency_s, p99_latency_s, max_latency_s, mean_batch_jobs;
+    double p50_latency_s, p99_latency_s, max_latency_s;
+    double mean_batch_jobs;          /* jobs per batched-kernel launch */
     double first_arrival_s, last_completion_s;  /* on the server's clock */
+    uint64_t batched_jobs;           /* jobs run in the batched kernel */
+    uint64_t direct_jobs;            /* jobs run as plain calls */
 } ars_server_stats;
 
 int ars_server_create(ars_ctx *ctx, const ars_server_config *cfg, ars_server **out);

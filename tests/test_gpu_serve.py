@@ -84,8 +84,17 @@ def test_server_alone_equals_in_mix(ars, direct_min_bytes, min_batch_jobs):
         assert st >= 0 and lat > 0
         assert np.array_equal(outs[i], alone[i]), i
     stats = srv.stats()
-    assert stats["completed"] == 2 * len(specs)
-    assert stats["batches"] < 2 * len(specs)  # the mix was batched
+    total = 2 * len(specs)
+    assert stats["completed"] == total
+    assert stats["batched_jobs"] + stats["direct_jobs"] == total
+    if direct_min_bytes == 1:  # every job a plain call: the batched kernel never ran
+        assert stats["batches"] == 0 and stats["direct_jobs"] == total
+    elif min_batch_jobs == 1 and direct_min_bytes == 2 ** 64 - 1:  # every job in the batched kernel
+        assert stats["direct_jobs"] == 0 and stats["batched_jobs"] == total
+        assert stats["batches"] < total  # the mix shared launches
+        assert stats["mean_batch_jobs"] == pytest.approx(total / stats["batches"])
+    elif min_batch_jobs == 1:  # split by codeword size: both paths ran
+        assert stats["batched_jobs"] > 0 and stats["direct_jobs"] > 0
     srv.close()
 
 
