@@ -165,10 +165,11 @@ int ars_encode(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *d_message, u
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const uint32_t T = n - k;
-    const bool al = (L % 4 == 0) && aligned4(d_message) && aligned4(d_parity);
+    const int am = ars::io_mode(d_message, d_parity, L);
+    const bool al = am == 1;
     if (is_pow2(T)) {
         const int tl = ilog2(T);
-        e = ars::launch_encode_ref(tl, al, (flags & ARS_ENC_ACCUMULATE) != 0, d_message, d_parity,
+        e = ars::launch_encode_ref(tl, am, (flags & ARS_ENC_ACCUMULATE) != 0, d_message, d_parity,
                                    L, k / T, ctx->d_tw, s, ctx->tune_pq[0][tl], ctx->tune_nbuf[0][tl]);
     } else {
         if (flags & ARS_ENC_ACCUMULATE)
@@ -220,9 +221,9 @@ int ars_recover_with_plan(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_recei
     if (rc) return rc;
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    const bool al = (L % 4 == 0) && aligned4(d_received);
+    const int am = ars::io_mode(d_received, d_received, L);
     const int nl = ilog2(n);
-    e = ars::launch_recover(nl, al, d_received, L, static_cast<const ars::RecPrep *>(d_plan),
+    e = ars::launch_recover(nl, am, d_received, L, static_cast<const ars::RecPrep *>(d_plan),
                             ctx->d_tw, static_cast<hipStream_t>(stream), ctx->tune_pq[1][nl], ctx->tune_nbuf[1][nl],
                             (uint32_t)ctx->num_cus);
     if (e != hipSuccess) return hip_fail(e, "recover launch");
@@ -243,18 +244,18 @@ int ars_recover(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const bool al = (L % 4 == 0) && aligned4(d_received);
+    const int am = ars::io_mode(d_received, d_received, L);
     const int nl = ilog2(n);
     // Launches of at most one workgroup per compute unit compute the plan in
     // the recover kernel itself: one launch instead of two.
-    e = ars::launch_recover_fused(nl, al, d_received, L, pa, e_cnt, ctx->d_gtab, ctx->d_tw, s,
+    e = ars::launch_recover_fused(nl, am, d_received, L, pa, e_cnt, ctx->d_gtab, ctx->d_tw, s,
                                   (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256));
     if (e == hipSuccess) return ARS_TRUE;
     if (e != hipErrorNotSupported) return hip_fail(e, "recover launch");
     auto enqueue = [&](ars::RecPrep *prep) {
         hipError_t le = ars::launch_recover_prep(prep, n, e_cnt, pa, ctx->d_gtab, s);
         if (le == hipSuccess)
-            le = ars::launch_recover(nl, al, d_received, L, prep, ctx->d_tw, s, ctx->tune_pq[1][nl],
+            le = ars::launch_recover(nl, am, d_received, L, prep, ctx->d_tw, s, ctx->tune_pq[1][nl],
                                      ctx->tune_nbuf[1][nl], (uint32_t)ctx->num_cus);
         return le;
     };
@@ -287,14 +288,14 @@ int ars_recover_noalloc(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_receive
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const bool al = (L % 4 == 0) && aligned4(d_received);
+    const int am = ars::io_mode(d_received, d_received, L);
     const int nl = ilog2(n);
-    e = ars::launch_recover_fused(nl, al, d_received, L, pa, e_cnt, ctx->d_gtab, ctx->d_tw, s,
+    e = ars::launch_recover_fused(nl, am, d_received, L, pa, e_cnt, ctx->d_gtab, ctx->d_tw, s,
                                   (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256));
     if (e == hipSuccess) return ARS_TRUE;
     if (e != hipErrorNotSupported) return hip_fail(e, "recover launch");
     if ((e = ars::launch_recover_prep(ctx->noalloc_plan, n, e_cnt, pa, ctx->d_gtab, s)) != hipSuccess ||
-        (e = ars::launch_recover(nl, al, d_received, L, ctx->noalloc_plan, ctx->d_tw, s, ctx->tune_pq[1][nl],
+        (e = ars::launch_recover(nl, am, d_received, L, ctx->noalloc_plan, ctx->d_tw, s, ctx->tune_pq[1][nl],
                                  ctx->tune_nbuf[1][nl], (uint32_t)ctx->num_cus)) != hipSuccess)
         return hip_fail(e, "recover launch");
     return ARS_TRUE;

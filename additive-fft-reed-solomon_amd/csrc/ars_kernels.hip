@@ -53,7 +53,7 @@ __device__ __forceinline__ int tile_q() {
     return q;
 }
 
-template <int TL, bool AL, bool ACC>
+template <int TL, int AM, bool ACC>
 __global__ void __launch_bounds__(256) k_encode_small(const uint8_t *__restrict__ msg,
                                                       uint8_t *__restrict__ par, uint32_t L,
                                                       uint32_t chunks,
@@ -62,13 +62,13 @@ __global__ void __launch_bounds__(256) k_encode_small(const uint8_t *__restrict_
     __shared__ MulTab tw[kTwiddleEntries];
     stage_twiddles<256>(tw, g_tw);
     __syncthreads();
-    BufIO<AL> io = make_bufio<AL>(msg, chunks * T * L, par, T * L, blockIdx.x * blockDim.x + threadIdx.x, L);
+    BufIO<AM> io = make_bufio<AM>(msg, chunks * T * L, par, T * L, blockIdx.x * blockDim.x + threadIdx.x, L);
     body_encode_small<TL, ACC>(io, chunks, tw);
 }
 
 // ONE: the message is a single chunk (k == T), known at compile time, so the
 // chunk loop and its accumulator disappear (fewer VGPRs, higher occupancy).
-template <int TL, int PQ, bool AL, bool ACC, int NBUF, bool ONE>
+template <int TL, int PQ, int AM, bool ACC, int NBUF, bool ONE>
 __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
     k_encode_tile(const uint8_t *__restrict__ msg, uint8_t *__restrict__ par, uint32_t L,
                   uint32_t chunks, const MulTab *__restrict__ g_tw) {
@@ -76,7 +76,7 @@ __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
     __shared__ uint32_t tile[NBUF * TT::WORDS];
     __shared__ MulTab tw[kTwiddleEntries];
     const int p = threadIdx.x % PQ, q = tile_q<PQ>();
-    BufIO<AL> io = make_bufio<AL>(msg, chunks * TT::N * L, par, TT::N * L, blockIdx.x * PQ + p, L);
+    BufIO<AM> io = make_bufio<AM>(msg, chunks * TT::N * L, par, TT::N * L, blockIdx.x * PQ + p, L);
     // the first chunk's rows are requested before the twiddles are staged, so
     // the two latencies overlap (measured faster for these 4-wave workgroups)
     uint32_t x0[16];
@@ -84,7 +84,7 @@ __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
     for (int j = 0; j < 16; ++j) x0[j] = io.ld_in((uint32_t)(q * 16), j);
     stage_twiddles<TT::THREADS>(tw, g_tw);
     __syncthreads();
-    body_encode_tile<TL, PQ, ACC, NBUF, BufIO<AL>, true>(io, q, p, tile, ONE ? 1u : chunks, tw, x0, g_tw);
+    body_encode_tile<TL, PQ, ACC, NBUF, BufIO<AM>, true>(io, q, p, tile, ONE ? 1u : chunks, tw, x0, g_tw);
 }
 
 template <int KL, bool AL>
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(256) k_recover_prep(RecPrep *__restrict__ out,
     recover_prep_body(out, n, e, pos.p, gtab);
 }
 
-template <int NL, bool AL>
+template <int NL, int AM>
 __global__ void __launch_bounds__(256) k_recover_small(uint8_t *__restrict__ data, uint32_t L,
                                                        const RecPrep *__restrict__ prep,
                                                        const MulTab *__restrict__ g_tw) {
@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(256) k_recover_small(uint8_t *__restrict__ dat
     stage_twiddles<256>(tw, g_tw);
     stage_prep<256, N>(s_tab, s_er, prep);
     __syncthreads();
-    BufIO<AL> io = make_bufio<AL>(data, N * L, data, N * L, blockIdx.x * blockDim.x + threadIdx.x, L);
+    BufIO<AM> io = make_bufio<AM>(data, N * L, data, N * L, blockIdx.x * blockDim.x + threadIdx.x, L);
     LdsTabs tabs{s_tab, s_er};
     body_recover_small<NL>(io, tabs, tw);
 }
@@ -164,7 +164,7 @@ __global__ void __launch_bounds__((Tile<NL, PQ>::THREADS))
 // instead of a second launch (the plan kernel) ahead of this one.  The tile
 // form keeps the plan's scratch in its tile buffer, idle until the first
 // exchange.
-template <int NL, bool AL>
+template <int NL, int AM>
 __global__ void __launch_bounds__(256) k_recover_small_fused(uint8_t *__restrict__ data, uint32_t L, PosArg pos,
                                                              uint32_t e, const uint8_t *__restrict__ gtab,
                                                              const MulTab *__restrict__ g_tw) {
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(256) k_recover_small_fused(uint8_t *__restrict
                       PlanScratch{s_scratch, s_scratch + 512, s_scratch + 768, s_scratch + 1024,
                                   reinterpret_cast<uint32_t *>(s_scratch + 1280)});
     __syncthreads();
-    BufIO<AL> io = make_bufio<AL>(data, N * L, data, N * L, blockIdx.x * blockDim.x + threadIdx.x, L);
+    BufIO<AM> io = make_bufio<AM>(data, N * L, data, N * L, blockIdx.x * blockDim.x + threadIdx.x, L);
     LdsTabs tabs{s_tab, s_er};
     body_recover_small<NL>(io, tabs, tw);
 }
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__((Tile<NL, PQ>::THREADS))
 // tiles blockIdx.x, blockIdx.x + gridDim.x, ...  The next tile's input rows
 // are requested once the current tile is back in the phase-A layout, so their
 // latency overlaps the phase-A FFT and the output of the current one.
-template <int NL, int PQ, bool OOB>
+template <int NL, int PQ, bool OOB, int AM>
 __device__ __forceinline__ void recover_stream(uint8_t *data, uint32_t L, uint32_t tiles, const MulTab *s_tab,
                                                const uint8_t *s_er, uint32_t *tile, const MulTab *tw) {
     using TT = Tile<NL, PQ>;
@@ -221,7 +221,7 @@ __device__ __forceinline__ void recover_stream(uint8_t *data, uint32_t L, uint32
     const Flags16 f = group_flags<PQ % 64 == 0>(tabs, q);  // the same as UNI below
     uint32_t t = blockIdx.x;
     constexpr bool UNI = PQ % 64 == 0;
-    BufIO<true, UNI> io = make_bufio<true, UNI>(data, N * L, data, N * L, t * PQ + p, L);
+    BufIO<AM, UNI> io = make_bufio<AM, UNI>(data, N * L, data, N * L, t * PQ + p, L);
     uint32_t rows[16];
     recover_rows_in<OOB>(io, f, q, rows);
     for (;;) {
@@ -230,16 +230,16 @@ __device__ __forceinline__ void recover_stream(uint8_t *data, uint32_t L, uint32
         // requested, so it is not live across phase B
         body_recover_tile1<NL, PQ>(io, tabs, f, q, p, tile, tw, rows, [&] {
             if (tn < tiles)
-                recover_rows_in<OOB>(make_bufio<true, UNI>(data, N * L, data, N * L, tn * PQ + p, L), f, q, rows);
+                recover_rows_in<OOB>(make_bufio<AM, UNI>(data, N * L, data, N * L, tn * PQ + p, L), f, q, rows);
         });
         if (tn >= tiles) break;
-        io = make_bufio<true, UNI>(data, N * L, data, N * L, tn * PQ + p, L);
+        io = make_bufio<AM, UNI>(data, N * L, data, N * L, tn * PQ + p, L);
         t = tn;
         __syncthreads();  // every lane's get_a of this tile is done before the next put_a
     }
 }
 
-template <int NL, int PQ, bool OOB>
+template <int NL, int PQ, bool OOB, int AM>
 __global__ void __launch_bounds__((Tile<NL, PQ>::THREADS)) __attribute__((amdgpu_waves_per_eu(4)))
     k_recover_stream(uint8_t *__restrict__ data, uint32_t L, uint32_t tiles, const RecPrep *__restrict__ prep,
                      const MulTab *__restrict__ g_tw) {
@@ -252,12 +252,12 @@ __global__ void __launch_bounds__((Tile<NL, PQ>::THREADS)) __attribute__((amdgpu
     stage_twiddles<TT::THREADS>(tw, g_tw);
     stage_prep<TT::THREADS, N>(s_tab, s_er, prep);
     __syncthreads();
-    recover_stream<NL, PQ, OOB>(data, L, tiles, s_tab, s_er, tile, tw);
+    recover_stream<NL, PQ, OOB, AM>(data, L, tiles, s_tab, s_er, tile, tw);
 }
 
 // The same with the erasure plan computed by each workgroup (launches of at
 // most one workgroup per compute unit; see k_recover_tile_fused).
-template <int NL, int PQ, bool OOB>
+template <int NL, int PQ, bool OOB, int AM>
 __global__ void __launch_bounds__((Tile<NL, PQ>::THREADS)) __attribute__((amdgpu_waves_per_eu(4)))
     k_recover_stream_fused(uint8_t *__restrict__ data, uint32_t L, uint32_t tiles, PosArg pos, uint32_t e,
                            const uint8_t *__restrict__ gtab, const MulTab *__restrict__ g_tw) {
@@ -273,7 +273,7 @@ __global__ void __launch_bounds__((Tile<NL, PQ>::THREADS)) __attribute__((amdgpu
     recover_plan<TT::THREADS>(PlanToLds{s_tab, s_er}, N, e, pos.p, gtab,
                               PlanScratch{sc, sc + 512, sc + 768, sc + 1024, reinterpret_cast<uint32_t *>(sc + 1280)});
     __syncthreads();  // the plan is in s_tab / s_er, and the tile buffer is free again
-    recover_stream<NL, PQ, OOB>(data, L, tiles, s_tab, s_er, tile, tw);
+    recover_stream<NL, PQ, OOB, AM>(data, L, tiles, s_tab, s_er, tile, tw);
 }
 
 template <int TL, bool AL>
@@ -316,21 +316,28 @@ static bool variant_ok(int tl, int pq, int nbuf) {
     return ((1 << (tl - 4)) * pq) <= 1024;
 }
 
-template <int TL, bool AL, bool ACC>
+// Column quads a launch covers: access mode 2 starts r bytes before the
+// first column (see BufIO), so the quads span r + L bytes.
+static inline uint64_t launch_quads(int am, const void *base, uint32_t L) {
+    const uint64_t r = am == 2 ? (reinterpret_cast<uintptr_t>(base) & 3u) : 0u;
+    return ((uint64_t)L + r + 3) / 4;
+}
+
+template <int TL, int AM, bool ACC>
 static hipError_t enc_one(const uint8_t *msg, uint8_t *par, uint32_t L, uint32_t chunks,
                           const MulTab *tw, hipStream_t s, int pq, int nbuf) {
-    const uint64_t quads = ((uint64_t)L + 3) / 4;
+    const uint64_t quads = launch_quads(AM, msg, L);
     if constexpr (TL <= 4) {
-        hipLaunchKernelGGL((k_encode_small<TL, AL, ACC>), dim3(ceil_div(quads, 256)), dim3(256), 0, s,
+        hipLaunchKernelGGL((k_encode_small<TL, AM, ACC>), dim3(ceil_div(quads, 256)), dim3(256), 0, s,
                            msg, par, L, chunks, tw);
     } else {
 #define ENC_V(PQ_, NB_)                                                                              \
     if (pq == PQ_ && nbuf == NB_) {                                                                  \
         if (chunks == 1)                                                                             \
-            hipLaunchKernelGGL((k_encode_tile<TL, PQ_, AL, ACC, NB_, true>), dim3(ceil_div(quads, PQ_)), \
+            hipLaunchKernelGGL((k_encode_tile<TL, PQ_, AM, ACC, NB_, true>), dim3(ceil_div(quads, PQ_)), \
                                dim3(Tile<TL, PQ_>::THREADS), 0, s, msg, par, L, chunks, tw);         \
         else                                                                                         \
-            hipLaunchKernelGGL((k_encode_tile<TL, PQ_, AL, ACC, NB_, false>), dim3(ceil_div(quads, PQ_)), \
+            hipLaunchKernelGGL((k_encode_tile<TL, PQ_, AM, ACC, NB_, false>), dim3(ceil_div(quads, PQ_)), \
                                dim3(Tile<TL, PQ_>::THREADS), 0, s, msg, par, L, chunks, tw);         \
         return hipGetLastError();                                                                    \
     }
@@ -342,29 +349,33 @@ static hipError_t enc_one(const uint8_t *msg, uint8_t *par, uint32_t L, uint32_t
 }
 
 template <int TL>
-static hipError_t enc_acc(bool al, bool acc, const uint8_t *msg, uint8_t *par, uint32_t L,
+static hipError_t enc_acc(int am, bool acc, const uint8_t *msg, uint8_t *par, uint32_t L,
                           uint32_t chunks, const MulTab *tw, hipStream_t s, int pq, int nbuf) {
-    if (al) return acc ? enc_one<TL, true, true>(msg, par, L, chunks, tw, s, pq, nbuf)
-                       : enc_one<TL, true, false>(msg, par, L, chunks, tw, s, pq, nbuf);
-    return acc ? enc_one<TL, false, true>(msg, par, L, chunks, tw, s, pq, nbuf)
-               : enc_one<TL, false, false>(msg, par, L, chunks, tw, s, pq, nbuf);
+    switch (am) {
+        case 1: return acc ? enc_one<TL, 1, true>(msg, par, L, chunks, tw, s, pq, nbuf)
+                           : enc_one<TL, 1, false>(msg, par, L, chunks, tw, s, pq, nbuf);
+        case 2: return acc ? enc_one<TL, 2, true>(msg, par, L, chunks, tw, s, pq, nbuf)
+                           : enc_one<TL, 2, false>(msg, par, L, chunks, tw, s, pq, nbuf);
+        default: return acc ? enc_one<TL, 0, true>(msg, par, L, chunks, tw, s, pq, nbuf)
+                            : enc_one<TL, 0, false>(msg, par, L, chunks, tw, s, pq, nbuf);
+    }
 }
 
-hipError_t launch_encode_ref(int tl, bool al, bool acc, const uint8_t *msg, uint8_t *par,
+hipError_t launch_encode_ref(int tl, int am, bool acc, const uint8_t *msg, uint8_t *par,
                              uint32_t L, uint32_t chunks, const MulTab *tw, hipStream_t s,
                              int pq, int nbuf) {
     if (pq == 0) pq = default_pq(0, tl);
     if (nbuf == 0) nbuf = default_nbuf(0, tl);
     if (!variant_ok(tl, pq, nbuf)) return hipErrorInvalidValue;
     switch (tl) {
-        case 0: return enc_acc<0>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
-        case 1: return enc_acc<1>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
-        case 2: return enc_acc<2>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
-        case 3: return enc_acc<3>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
-        case 4: return enc_acc<4>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
-        case 5: return enc_acc<5>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
-        case 6: return enc_acc<6>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
-        case 7: return enc_acc<7>(al, acc, msg, par, L, chunks, tw, s, pq, nbuf);
+        case 0: return enc_acc<0>(am, acc, msg, par, L, chunks, tw, s, pq, nbuf);
+        case 1: return enc_acc<1>(am, acc, msg, par, L, chunks, tw, s, pq, nbuf);
+        case 2: return enc_acc<2>(am, acc, msg, par, L, chunks, tw, s, pq, nbuf);
+        case 3: return enc_acc<3>(am, acc, msg, par, L, chunks, tw, s, pq, nbuf);
+        case 4: return enc_acc<4>(am, acc, msg, par, L, chunks, tw, s, pq, nbuf);
+        case 5: return enc_acc<5>(am, acc, msg, par, L, chunks, tw, s, pq, nbuf);
+        case 6: return enc_acc<6>(am, acc, msg, par, L, chunks, tw, s, pq, nbuf);
+        case 7: return enc_acc<7>(am, acc, msg, par, L, chunks, tw, s, pq, nbuf);
         default: return hipErrorInvalidValue;
     }
 }
@@ -403,8 +414,9 @@ hipError_t launch_recover_prep(RecPrep *out, uint32_t n, uint32_t e, const PosAr
 }
 
 // Erased rows may be read at the buffer's end (see recover_rows_in) when no
-// row offset added to it wraps past 2^32.
-static bool oob_ok(int nl, uint32_t L) { return ((uint64_t)(1u << nl) + 15u) * L < (1ull << 32); }
+// row offset added to it wraps past 2^32 (8 bytes of slack for access mode 2's
+// rounding).
+static bool oob_ok(int nl, uint32_t L) { return ((uint64_t)(1u << nl) + 15u) * L + 8u < (1ull << 32); }
 
 // Workgroups of `kernel` resident per compute unit (at least 1).
 template <class K>
@@ -419,30 +431,30 @@ static uint32_t resident_groups(K kernel, int threads) {
 // most one per tile.
 template <int NL, int PQ>
 static uint32_t stream_grid(uint32_t tiles, uint32_t cus) {
-    static const uint32_t occ = resident_groups(k_recover_stream<NL, PQ, true>, Tile<NL, PQ>::THREADS);
+    static const uint32_t occ = resident_groups(k_recover_stream<NL, PQ, true, 1>, Tile<NL, PQ>::THREADS);
     const uint64_t g = 2ull * occ * (cus ? cus : 256u);
     return (uint32_t)(g < tiles ? g : tiles);
 }
 
-template <int NL, bool AL>
+template <int NL, int AM>
 static hipError_t rec_one(uint8_t *data, uint32_t L, const RecPrep *prep, const MulTab *tw,
                           hipStream_t s, int pq, int nbuf, uint32_t cus) {
-    const uint64_t quads = ((uint64_t)L + 3) / 4;
+    const uint64_t quads = launch_quads(AM, data, L);
     if constexpr (NL <= 4) {
-        hipLaunchKernelGGL((k_recover_small<NL, AL>), dim3(ceil_div(quads, 256)), dim3(256), 0, s,
+        hipLaunchKernelGGL((k_recover_small<NL, AM>), dim3(ceil_div(quads, 256)), dim3(256), 0, s,
                            data, L, prep, tw);
         return hipGetLastError();
     } else {
-        if constexpr (AL) {
-            if (nbuf == 1) {
+        if constexpr (AM != 0) {
+            if (nbuf == 1 || AM == 2) {
 #define RS_V(PQ_)                                                                                    \
     if (pq == PQ_) {                                                                                 \
         const uint32_t tiles = ceil_div(quads, PQ_), grid = stream_grid<NL, PQ_>(tiles, cus);        \
         if (oob_ok(NL, L))                                                                           \
-            hipLaunchKernelGGL((k_recover_stream<NL, PQ_, true>), dim3(grid),                        \
+            hipLaunchKernelGGL((k_recover_stream<NL, PQ_, true, AM>), dim3(grid),                    \
                                dim3(Tile<NL, PQ_>::THREADS), 0, s, data, L, tiles, prep, tw);        \
         else                                                                                         \
-            hipLaunchKernelGGL((k_recover_stream<NL, PQ_, false>), dim3(grid),                       \
+            hipLaunchKernelGGL((k_recover_stream<NL, PQ_, false, AM>), dim3(grid),                   \
                                dim3(Tile<NL, PQ_>::THREADS), 0, s, data, L, tiles, prep, tw);        \
         return hipGetLastError();                                                                    \
     }
@@ -451,25 +463,31 @@ static hipError_t rec_one(uint8_t *data, uint32_t L, const RecPrep *prep, const 
                 return hipErrorInvalidValue;
             }
         }
+        if constexpr (AM != 2) {
+            constexpr bool AL = AM == 1;
 #define REC_V(PQ_, NB_)                                                                              \
     if (pq == PQ_ && nbuf == NB_) {                                                                  \
         hipLaunchKernelGGL((k_recover_tile<NL, PQ_, AL, NB_>), dim3(ceil_div(quads, PQ_)),           \
                            dim3(Tile<NL, PQ_>::THREADS), 0, s, data, L, prep, tw);                   \
         return hipGetLastError();                                                                    \
     }
-        REC_V(16, 1) REC_V(16, 2) REC_V(32, 1) REC_V(32, 2) REC_V(64, 1) REC_V(64, 2)
+            REC_V(16, 1) REC_V(16, 2) REC_V(32, 1) REC_V(32, 2) REC_V(64, 1) REC_V(64, 2)
 #undef REC_V
+        }
         return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_recover(int nl, bool al, uint8_t *data, uint32_t L, const RecPrep *prep,
+hipError_t launch_recover(int nl, int am, uint8_t *data, uint32_t L, const RecPrep *prep,
                           const MulTab *tw, hipStream_t s, int pq, int nbuf, uint32_t cus) {
-    if (pq == 0) pq = (al || nl < 8) ? default_pq(1, nl) : 32;  // byte path: 32 quads at 256 points
-    if (nbuf == 0) nbuf = al ? default_nbuf(1, nl) : 2;
+    if (pq == 0) pq = (am != 0 || nl < 8) ? default_pq(1, nl) : 32;  // byte path: 32 quads at 256 points
+    if (nbuf == 0) nbuf = am != 0 ? default_nbuf(1, nl) : 2;
     if (!variant_ok(nl, pq, nbuf)) return hipErrorInvalidValue;
-#define RC_CASE(K) \
-    case K: return al ? rec_one<K, true>(data, L, prep, tw, s, pq, nbuf, cus) : rec_one<K, false>(data, L, prep, tw, s, pq, nbuf, cus);
+#define RC_CASE(K)                                                                                  \
+    case K:                                                                                         \
+        return am == 1 ? rec_one<K, 1>(data, L, prep, tw, s, pq, nbuf, cus)                         \
+             : am == 2 ? rec_one<K, 2>(data, L, prep, tw, s, pq, nbuf, cus)                         \
+                       : rec_one<K, 0>(data, L, prep, tw, s, pq, nbuf, cus);
     switch (nl) {
         RC_CASE(1) RC_CASE(2) RC_CASE(3) RC_CASE(4) RC_CASE(5) RC_CASE(6) RC_CASE(7) RC_CASE(8)
         default: return hipErrorInvalidValue;
@@ -477,40 +495,41 @@ hipError_t launch_recover(int nl, bool al, uint8_t *data, uint32_t L, const RecP
 #undef RC_CASE
 }
 
-template <int NL, bool AL>
+template <int NL, int AM>
 static hipError_t rec_fused_one(uint8_t *data, uint32_t L, const PosArg &pos, uint32_t e, const uint8_t *gtab,
                                 const MulTab *tw, hipStream_t s, uint32_t max_groups) {
-    const uint64_t quads = ((uint64_t)L + 3) / 4;
+    const uint64_t quads = launch_quads(AM, data, L);
     if constexpr (NL <= 4) {
         const uint32_t grid = ceil_div(quads, 256);
         if (grid > max_groups) return hipErrorNotSupported;
-        hipLaunchKernelGGL((k_recover_small_fused<NL, AL>), dim3(grid), dim3(256), 0, s, data, L, pos, e, gtab, tw);
-    } else if constexpr (AL) {
+        hipLaunchKernelGGL((k_recover_small_fused<NL, AM>), dim3(grid), dim3(256), 0, s, data, L, pos, e, gtab, tw);
+    } else if constexpr (AM != 0) {
         constexpr int PQ = default_pq(1, NL);
         const uint32_t grid = ceil_div(quads, PQ);
         if (grid > max_groups) return hipErrorNotSupported;
         if (oob_ok(NL, L))
-            hipLaunchKernelGGL((k_recover_stream_fused<NL, PQ, true>), dim3(grid), dim3(Tile<NL, PQ>::THREADS), 0, s,
-                               data, L, grid, pos, e, gtab, tw);
+            hipLaunchKernelGGL((k_recover_stream_fused<NL, PQ, true, AM>), dim3(grid), dim3(Tile<NL, PQ>::THREADS), 0,
+                               s, data, L, grid, pos, e, gtab, tw);
         else
-            hipLaunchKernelGGL((k_recover_stream_fused<NL, PQ, false>), dim3(grid), dim3(Tile<NL, PQ>::THREADS), 0, s,
-                               data, L, grid, pos, e, gtab, tw);
+            hipLaunchKernelGGL((k_recover_stream_fused<NL, PQ, false, AM>), dim3(grid), dim3(Tile<NL, PQ>::THREADS),
+                               0, s, data, L, grid, pos, e, gtab, tw);
     } else {
         constexpr int PQ = NL >= 8 ? 32 : 64, NB = 2;  // the byte path's default variant
         const uint32_t grid = ceil_div(quads, PQ);
         if (grid > max_groups) return hipErrorNotSupported;
-        hipLaunchKernelGGL((k_recover_tile_fused<NL, PQ, AL, NB>), dim3(grid), dim3(Tile<NL, PQ>::THREADS), 0, s,
+        hipLaunchKernelGGL((k_recover_tile_fused<NL, PQ, false, NB>), dim3(grid), dim3(Tile<NL, PQ>::THREADS), 0, s,
                            data, L, pos, e, gtab, tw);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_recover_fused(int nl, bool al, uint8_t *data, uint32_t L, const PosArg &pos, uint32_t e,
+hipError_t launch_recover_fused(int nl, int am, uint8_t *data, uint32_t L, const PosArg &pos, uint32_t e,
                                 const uint8_t *gtab, const MulTab *tw, hipStream_t s, uint32_t max_groups) {
 #define RF_CASE(K)                                                                                  \
     case K:                                                                                         \
-        return al ? rec_fused_one<K, true>(data, L, pos, e, gtab, tw, s, max_groups)               \
-                  : rec_fused_one<K, false>(data, L, pos, e, gtab, tw, s, max_groups);
+        return am == 1 ? rec_fused_one<K, 1>(data, L, pos, e, gtab, tw, s, max_groups)              \
+             : am == 2 ? rec_fused_one<K, 2>(data, L, pos, e, gtab, tw, s, max_groups)              \
+                       : rec_fused_one<K, 0>(data, L, pos, e, gtab, tw, s, max_groups);
     switch (nl) {
         RF_CASE(1) RF_CASE(2) RF_CASE(3) RF_CASE(4) RF_CASE(5) RF_CASE(6) RF_CASE(7) RF_CASE(8)
         default: return hipErrorInvalidValue;

@@ -33,7 +33,9 @@ constexpr int default_nbuf(int op, int tl) { return op == 1 ? 1 : (tl >= 7 ? 1 :
 
 // pq / nbuf select a tile variant for transforms of 32+ points (0 = default):
 // pq = column quads per workgroup (16, 32 or 64), nbuf = LDS exchange buffers (1 or 2).
-hipError_t launch_encode_ref(int tl, bool aligned, bool accumulate, const uint8_t *msg, uint8_t *par,
+// am: access mode of the buffers (BufIO in ars_tiles.hpp: 0 bytes, 1 aligned
+// dwords, 2 dwords r bytes past alignment; io_mode() below picks it).
+hipError_t launch_encode_ref(int tl, int am, bool accumulate, const uint8_t *msg, uint8_t *par,
                              uint32_t L, uint32_t chunks, const MulTab *tw, hipStream_t s,
                              int pq = 0, int nbuf = 0);
 hipError_t launch_encode_kpow2(int kl, bool aligned, const uint8_t *msg, uint8_t *par, uint32_t L,
@@ -42,14 +44,23 @@ hipError_t launch_recover_prep(RecPrep *out, uint32_t n, uint32_t e, const PosAr
                                const uint8_t *gtab, hipStream_t s);
 // Recover with a ready plan.  With nbuf == 1 and dword-addressable rows this
 // is the streaming kernel, whose grid is sized from `cus` compute units.
-hipError_t launch_recover(int nl, bool aligned, uint8_t *data, uint32_t L, const RecPrep *prep,
+hipError_t launch_recover(int nl, int am, uint8_t *data, uint32_t L, const RecPrep *prep,
                           const MulTab *tw, hipStream_t s, int pq = 0, int nbuf = 0, uint32_t cus = 0);
 // Recover in one launch, each workgroup computing the plan itself (the
 // default tile variant), when the launch has at most max_groups workgroups;
 // hipErrorNotSupported (nothing launched) otherwise.
-hipError_t launch_recover_fused(int nl, bool aligned, uint8_t *data, uint32_t L, const PosArg &pos, uint32_t e,
+hipError_t launch_recover_fused(int nl, int am, uint8_t *data, uint32_t L, const PosArg &pos, uint32_t e,
                                 const uint8_t *gtab, const MulTab *tw, hipStream_t s, uint32_t max_groups);
 hipError_t launch_recompute(int tl, bool aligned, uint8_t *data, uint32_t L, const MulTab *tw,
                             hipStream_t s);
+
+// Access mode for buffers a and b (b = a for in-place work): 1 when both are
+// 4-byte aligned, 2 when both sit the same 1..3 bytes past alignment, else 0;
+// shard_len must be a multiple of 4 for 1 or 2.
+inline int io_mode(const void *a, const void *b, size_t L) {
+    const uintptr_t ra = reinterpret_cast<uintptr_t>(a) & 3u, rb = reinterpret_cast<uintptr_t>(b) & 3u;
+    if (L % 4 || ra != rb) return 0;
+    return ra ? 2 : 1;
+}
 
 }  // namespace ars

@@ -611,11 +611,26 @@ __device__ __forceinline__ void recover_prep_body(RecPrep *__restrict__ out, uin
 // I/O policies and multiplier sources.
 
 // One codeword per launch: buffer resources, scalar per-shard offsets.
-template <bool AL, bool UNI = false>
+// Access modes (AM):
+//   0  bytes: any base and shard_len; a quad is read and written byte by byte;
+//   1  dwords: shard_len % 4 == 0 and 4-byte aligned buffers;
+//   2  shifted dwords: shard_len % 4 == 0 and input and output both r = 1..3
+//      bytes past a dword boundary, so every row is too.  The rows are
+//      addressed from the dword boundary below the base, and a lane's quad is
+//      the dword that holds columns 4*quad - r .. 4*quad - r + 3 (columns are
+//      independent, so where a quad starts does not matter).  The first and
+//      last quad of a row hold bytes of the neighbouring rows or past the
+//      buffer; the transform computes them but the lane writes only the bytes
+//      of its own row (`bm`).
+// A bool AL converts to 0 or 1.
+template <int AM, bool UNI = false>
 struct BufIO {
+    static_assert(AM >= 0 && AM <= 2, "access mode");
+    static constexpr bool AL = AM != 0;           // dword loads
     static constexpr bool kUniformGroups = UNI;  // every lane of a wave holds one shard group
     Rows in, out;
     uint32_t L, off, nb, oob;  // oob: the input buffer's size (see ld_in_or0)
+    uint32_t bm;               // AM 2: the quad's bytes that belong to the row (bit per byte)
     __device__ __forceinline__ uint32_t ld_in(uint32_t g, int j) const {
         return ldq<AL>(in, off + g * L, (uint32_t)j * L, nb);
     }
@@ -623,31 +638,70 @@ struct BufIO {
         return ldq<AL>(out, off + g * L, (uint32_t)j * L, nb);
     }
     __device__ __forceinline__ void st_out(uint32_t g, int j, uint32_t v) const {
-        stq<AL>(out, off + g * L, (uint32_t)j * L, v, nb);
+        if constexpr (AM == 2) {
+            const uint32_t vo = off + g * L, so = (uint32_t)j * L;
+            if (bm == 0xfu) {
+                __builtin_amdgcn_raw_buffer_store_b32(v, out.r, (int)vo, (int)so, 0);
+            } else if (bm) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if ((bm >> i) & 1u)
+                        __builtin_amdgcn_raw_buffer_store_b8((uint8_t)(v >> (8 * i)), out.r, (int)(vo + i), (int)so, 0);
+            }
+        } else {
+            stq<AL>(out, off + g * L, (uint32_t)j * L, v, nb);
+        }
     }
     // input row g + j, or 0 if `zero` (read at `oob`, past the buffer's end)
     __device__ __forceinline__ uint32_t ld_in_or0(uint32_t g, int j, bool zero) const {
         static_assert(AL, "ld_in_or0 needs dword-addressable rows");
         return __builtin_amdgcn_raw_buffer_load_b32(in.r, (int)(zero ? oob : off + g * L), (int)((uint32_t)j * L), 0);
     }
-    // out ^= v (dword rows only)
+    // out ^= v (dword rows only; AM 2: the bytes outside the row XOR in 0)
     __device__ __forceinline__ void xor_out(uint32_t g, int j, uint32_t v) const {
         static_assert(AL, "xor_out needs dword-addressable rows");
-        if (nb) ars_buffer_atomic_xor((int)v, out.r, (int)(off + g * L), (int)((uint32_t)j * L), 0);
+        if constexpr (AM == 2) {
+            const uint32_t keep = (bm & 1u ? 0xffu : 0u) | (bm & 2u ? 0xff00u : 0u) | (bm & 4u ? 0xff0000u : 0u) |
+                                  (bm & 8u ? 0xff000000u : 0u);
+            if (bm) ars_buffer_atomic_xor((int)(v & keep), out.r, (int)(off + g * L), (int)((uint32_t)j * L), 0);
+        } else {
+            if (nb) ars_buffer_atomic_xor((int)v, out.r, (int)(off + g * L), (int)((uint32_t)j * L), 0);
+        }
     }
 };
 
-template <bool AL, bool UNI = false>
-__device__ __forceinline__ BufIO<AL, UNI> make_bufio(const void *in, uint32_t in_bytes, void *out,
+// Byte offset of p past the dword boundary below it (access mode 2's r).
+__device__ __forceinline__ uint32_t dword_shift(const void *p) { return (uint32_t)(reinterpret_cast<uintptr_t>(p) & 3u); }
+
+template <int AM, bool UNI = false>
+__device__ __forceinline__ BufIO<AM, UNI> make_bufio(const void *in, uint32_t in_bytes, void *out,
                                                      uint32_t out_bytes, uint32_t quad, uint32_t L) {
-    BufIO<AL, UNI> io;
-    io.in = make_rows(in, in_bytes);
-    io.out = make_rows(out, out_bytes);
+    BufIO<AM, UNI> io;
     io.L = L;
-    io.oob = in_bytes;
     const uint32_t col = quad * 4u;
-    io.nb = col < L ? ((L - col) < 4u ? (L - col) : 4u) : 0u;
-    io.off = (AL && io.nb == 0u) ? 0u : col;
+    if constexpr (AM == 2) {
+        // rows addressed from the dword boundary below each base; both bases
+        // have the same r (the launchers check it)
+        const uint32_t r = dword_shift(in);
+        const uint32_t inb = (in_bytes + r + 3u) & ~3u, outb = (out_bytes + r + 3u) & ~3u;
+        io.in = make_rows(static_cast<const uint8_t *>(in) - r, inb);
+        io.out = make_rows(static_cast<uint8_t *>(out) - r, outb);
+        io.oob = inb;
+        // columns c0 .. c0 + 3 of the row, c0 = col - r; keep those in [0, L)
+        const int c0 = (int)col - (int)r;
+        const int lo = c0 < 0 ? -c0 : 0;
+        const int hi = (int)L - c0 < 4 ? (int)L - c0 : 4;
+        io.bm = hi > lo ? ((1u << hi) - 1u) & ~((1u << lo) - 1u) : 0u;
+        io.nb = io.bm ? 4u : 0u;
+        io.off = io.bm ? col : 0u;
+    } else {
+        io.in = make_rows(in, in_bytes);
+        io.out = make_rows(out, out_bytes);
+        io.oob = in_bytes;
+        io.bm = 0;
+        io.nb = col < L ? ((L - col) < 4u ? (L - col) : 4u) : 0u;
+        io.off = (AM == 1 && io.nb == 0u) ? 0u : col;
+    }
     return io;
 }
 

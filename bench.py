@@ -36,12 +36,31 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(n, k, L, e, seconds, threads=16):
+def cpu_info():
+    """Host CPU as /proc/cpuinfo and the scheduler report it: model string,
+    logical CPUs online, CPUs this process may run on, and the flags the
+    restatement needs."""
+    flags = set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("flags"):
+                flags = set(line.split(":", 1)[1].split())
+                break
+    except OSError:
+        pass
+    return {"model": cpu_model(), "nproc_online": os.cpu_count(),
+            "usable_cpus": len(os.sched_getaffinity(0)),
+            "avx512bw": "avx512bw" in flags, "gfni": "gfni" in flags}
+
+
+def cpu_baseline(n, k, L, e, seconds):
     """Oracle restatement of the reference's CPU path on the same (n, k, L)
     codeword: encode + recover steps until `seconds` have passed, on one core
-    (the reference is single-threaded) and then on `threads` cores with one
-    independent codeword per thread.  Also times the reference README's
-    (64, 32) rows (README.md:29-35, AMD EPYC 9575F) for a sanity check."""
+    (the reference is single-threaded) and then on every CPU this process may
+    use, one pinned thread and one independent codeword each.  Also times the
+    reference README's (64, 32) rows (README.md:29-35, AMD EPYC 9575F) for a
+    sanity check, through the reference's beta = 0 fixed sequences (its
+    default dispatch) and through the generic recursion."""
     import numpy as np
     from concurrent.futures import ThreadPoolExecutor
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -50,7 +69,9 @@ def cpu_baseline(n, k, L, e, seconds, threads=16):
     gfni = O.gfni_supported()
     step_bytes = n * L + (n + e) * L
 
-    def worker(seed, budget):
+    def worker(seed, budget, cpu=None):
+        if cpu is not None:
+            os.sched_setaffinity(0, {cpu})  # this thread only (Linux: 0 = the caller)
         rng = np.random.default_rng(seed)
         msg = rng.integers(0, 256, k * L, dtype=np.uint8)
         pos = np.sort(rng.permutation(n)[:e]).astype(np.uint8)
@@ -78,24 +99,32 @@ def cpu_baseline(n, k, L, e, seconds, threads=16):
             steps += 1
         return steps, t_enc, t_rec
 
-    steps, t_enc, t_rec = worker(7, seconds)
-    threads = max(1, min(threads, os.cpu_count() or 1))
+    cpus = sorted(os.sched_getaffinity(0))
+    main_aff = set(cpus)
+    steps, t_enc, t_rec = worker(7, seconds, cpus[0])
+    os.sched_setaffinity(0, main_aff)
+    threads = len(cpus)
     t0 = time.perf_counter()
     with ThreadPoolExecutor(threads) as ex:  # ctypes calls release the GIL
-        res = list(ex.map(lambda i: worker(100 + i, seconds / 2), range(threads)))
+        res = list(ex.map(lambda i: worker(100 + i, seconds / 2, cpus[i]), range(threads)))
     wall = time.perf_counter() - t0
     all_steps = sum(r[0] for r in res)
     readme = {}
     for Lr, enc_ref, rec_ref in ((64, "385ns, 10GiB/s", "7.7us, 510MiB/s"), (1024, "2.4us, 25GiB/s", "14us, 4.1GiB/s"),
                                  (65536, "192us, 20GiB/s", "740us, 5.3GiB/s")):
         pos = np.sort(np.random.default_rng(Lr).permutation(64)[:32]).astype(np.uint8)
-        t = O.gfni_time_codec(64, 32, Lr, pos, max(5, int(4e7 / (64 * Lr))))
-        if t is None:
-            break
-        me, mr = t[0] * 1e-9, t[1] * 1e-9
-        readme[str(Lr)] = {"encode_us": round(me * 1e6, 2), "encode_GiBps": round(64 * Lr / me / 2**30, 2),
-                           "recover_us": round(mr * 1e6, 2), "recover_GiBps": round(64 * Lr / mr / 2**30, 2),
-                           "readme_encode": enc_ref, "readme_recover": rec_ref}
+        row = {"readme_encode": enc_ref, "readme_recover": rec_ref}
+        for form, unrolled in (("", True), ("generic_", False)):
+            O.gfni_set_unrolled(unrolled)
+            t = O.gfni_time_codec(64, 32, Lr, pos, max(5, int(4e7 / (64 * Lr))))
+            if t is None:
+                break
+            me, mr = t[0] * 1e-9, t[1] * 1e-9
+            row.update({f"{form}encode_us": round(me * 1e6, 2), f"{form}encode_GiBps": round(64 * Lr / me / 2**30, 2),
+                        f"{form}recover_us": round(mr * 1e6, 2),
+                        f"{form}recover_GiBps": round(64 * Lr / mr / 2**30, 2)})
+        O.gfni_set_unrolled(True)
+        readme[str(Lr)] = row
     return {
         "value": round(steps * step_bytes / (t_enc + t_rec) / 1e9, 3),
         "unit": "GB/s",
@@ -107,29 +136,41 @@ def cpu_baseline(n, k, L, e, seconds, threads=16):
         "encode_GBps": round(steps * n * L / t_enc / 1e9, 3),
         "recover_GBps": round(steps * (n + e) * L / t_rec / 1e9, 3),
         "recover_GBps_nL_convention": round(steps * n * L / t_rec / 1e9, 3),
-        "all_cores": {"threads": threads, "steps": all_steps,
+        "host": cpu_info(),
+        "all_cores": {"threads": threads, "pinned": True, "steps": all_steps,
                       "GBps": round(all_steps * step_bytes / wall / 1e9, 2),
-                      "sample": "one independent codeword per thread, same shape"},
+                      "sample": "one independent codeword per usable CPU, each thread pinned to its CPU, same shape"},
         "readme_64x32_check": readme,
     }
 
 
+def device_source_sha():
+    """SHA-256 over the device sources (csrc/*.hip and *.hpp, sorted by name):
+    a committed profile applies only to kernels built from the same text."""
+    import hashlib
+    csrc = os.path.join(ROOT, "additive-fft-reed-solomon_amd", "csrc")
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(csrc)):
+        if name.endswith((".hip", ".hpp")):
+            h.update(name.encode())
+            with open(os.path.join(csrc, name), "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()
+
+
 def _profile_entry(fname, kernel):
-    """The committed rocprofv3 record (profiles/<fname>) of `kernel`, or of
-    the same kernel family (another tile variant) when that variant was not
-    profiled; (name, record) or (None, None)."""
+    """The committed rocprofv3 record (profiles/<fname>) of exactly `kernel`,
+    provided the record was made from the device sources this run uses
+    (device_source_sha); (name, record) or (None, None)."""
     try:
         with open(os.path.join(ROOT, "profiles", fname)) as f:
-            recs = json.load(f)["kernels"]
+            doc = json.load(f)
+        recs = doc["kernels"]
     except (OSError, ValueError, KeyError):
         return None, None
-    if kernel in recs:
-        return kernel, recs[kernel]
-    fam = kernel.split(",")[0]
-    for name, rec in recs.items():
-        if name.startswith(fam):
-            return name, rec
-    return None, None
+    if kernel not in recs or doc.get("device_source_sha") != device_source_sha():
+        return None, None
+    return kernel, recs[kernel]
 
 
 def measured_traffic(kernel):
@@ -473,6 +514,179 @@ def bench_served_small(ars, torch, np, rates=(20000, 50000, 100000, 200000), per
             "sweep": out}
 
 
+# (N, T) pairs of the reference's criterion benches, each at shard_len 64,
+# 1024 and 65536, aligned and one byte past alignment
+# (rs/benches/encode_systematic_sharded.rs:103-160,
+# rs/benches/recover_erasure_shards.rs:158-215)
+GRID_NT = [(2, 1), (4, 1), (4, 2), (8, 2), (8, 4), (16, 4), (16, 8), (32, 8), (32, 16), (64, 16), (64, 32),
+           (128, 32), (128, 64), (256, 64), (256, 128)]
+GRID_L = (64, 1024, 65536)
+
+
+def _per_call(torch, stream, fn, budget_s=0.004, sync_reps=15, min_reps=5):
+    """(mean us per call issued back to back on `stream`, median us of one
+    call waited for on the host).  The first is the device-side cost once
+    launches overlap; the second what a caller that waits for each call sees."""
+    fn()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    one = max(time.perf_counter() - t, 1e-6)
+    reps = int(min(200, max(min_reps, budget_s / one)))
+    a, z = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        fn()
+    z.record(stream)
+    torch.cuda.synchronize()
+    stream_us = a.elapsed_time(z) / reps * 1e3
+    lat = [one]
+    for _ in range(sync_reps - 1):
+        t = time.perf_counter()
+        fn()
+        stream.synchronize()
+        lat.append(time.perf_counter() - t)
+    return stream_us, sorted(lat)[len(lat) // 2] * 1e6
+
+
+def bench_reference_grid(ars, torch, np, stream, with_cpu=True, seed=42):
+    """The reference's benchmark grid on the GPU: per-call encode and recover
+    (T random erasures, zero-filled, as rs/benches/recover_erasure_shards.rs:
+    110-134) at every (N, T, shard_len) point, device buffers aligned and one
+    byte past a 256-byte boundary, beside the CPU restatement at the same
+    points (median of C-timed calls, one thread).  Every GPU point is checked:
+    the recovered codeword equals the encoded one."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    ctx = ars.default_context(0)
+    cap = 256 * 65536 + 512
+    mbuf = torch.empty(cap, dtype=torch.uint8, device="cuda")
+    cbuf = torch.empty(cap, dtype=torch.uint8, device="cuda")
+    rows = []
+    gfni = with_cpu and O.gfni_supported()
+    for L in GRID_L:
+        for N, T in GRID_NT:
+            k = N - T
+            rng = np.random.default_rng(seed + N * 1000 + T * 10 + L)
+            pos = np.sort(rng.permutation(N)[:T]).astype(np.uint8)
+            row = {"N": N, "T": T, "L": L}
+            for aligned in (True, False):
+                off = 0 if aligned else 1
+                msg = mbuf[off:off + k * L]
+                cw = cbuf[off:off + N * L]
+                msg.copy_(torch.from_numpy(rng.integers(0, 256, k * L, dtype=np.uint8)))
+                cw.zero_()
+                ctx.encode(N, k, msg.data_ptr(), cw.data_ptr(), L, 0, stream)
+                cw[T * L:] = msg
+                torch.cuda.synchronize()
+                good = cw.clone()
+                cw.view(N, L)[torch.from_numpy(pos.astype(np.int64)).cuda()] = 0
+                torch.cuda.synchronize()
+                ctx.recover(N, k, cw.data_ptr(), L, pos, stream)
+                torch.cuda.synchronize()
+                if not torch.equal(cw, good):
+                    raise RuntimeError(f"grid point N={N} T={T} L={L} aligned={aligned}: recover differs")
+                par = cbuf[off:off + T * L]
+                e_s, e_l = _per_call(torch, stream, lambda: ctx.encode(N, k, msg.data_ptr(), par.data_ptr(), L, 0,
+                                                                       stream))
+                # repeated in place: each call XORs its result into the shards again
+                # (data-independent time)
+                r_s, r_l = _per_call(torch, stream, lambda: ctx.recover(N, k, cw.data_ptr(), L, pos, stream))
+                tag = "aligned" if aligned else "unaligned"
+                row[tag] = {"encode_us": round(e_s, 2), "encode_sync_us": round(e_l, 2),
+                            "recover_us": round(r_s, 2), "recover_sync_us": round(r_l, 2)}
+                if gfni:
+                    reps = int(min(400, max(5, 3e7 / (N * L))))
+                    t = O.gfni_time_codec(N, T, L, pos, reps, off)
+                    if t is not None:
+                        row[tag]["cpu_encode_us"] = round(t[0] / 1e3, 2)
+                        row[tag]["cpu_recover_us"] = round(t[1] / 1e3, 2)
+            rows.append(row)
+    del mbuf, cbuf
+
+    def ratio(key, gkey):
+        v = [r[a][key] / r[a][gkey] for r in rows for a in ("aligned", "unaligned") if key in r[a]]
+        return round(float(np.median(v)), 2) if v else None
+    return {"workload": "the reference's criterion grid: 15 (N,T) x shard_len {64, 1024, 65536} x {aligned, "
+                        "+1 byte}; encode of random messages, recover of T random zero-filled erasures",
+            "cpu": (f"AVX-512BW+GFNI restatement, 1 thread, {cpu_model()}" if gfni else None),
+            "median_cpu_over_gpu": {"encode": ratio("cpu_encode_us", "encode_us"),
+                                    "recover": ratio("cpu_recover_us", "recover_us")},
+            "points": rows}
+
+
+def bench_c5(ars, torch, np, stream, seed=46):
+    """The adversarial shapes: (256,224) and (256,16) with 4 MiB shards --
+    encode and recover GB/s and HBM fraction under all-parity,
+    all-leading-data, contiguous-run and alternating erasures -- and 65536
+    (32,16) codewords with 64 B shards, one call each vs one batch."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from test_gpu_adversarial import patterns
+    ctx = ars.default_context(0)
+    rng = np.random.default_rng(seed)
+    L = 4 << 20
+    shapes = {}
+    for n, k in ((256, 224), (256, 16)):
+        T = n - k
+        cw = torch.randint(0, 256, (n * L,), dtype=torch.uint8, device="cuda")
+        par, msg = cw[:T * L], cw[T * L:]
+        ctx.encode(n, k, msg.data_ptr(), par.data_ptr(), L, 0, stream)
+        torch.cuda.synchronize()
+        good = cw.clone()
+        enc_s, _ = _per_call(torch, stream, lambda: ctx.encode(n, k, msg.data_ptr(), par.data_ptr(), L, 0, stream),
+                             budget_s=0.05, sync_reps=3)
+        res = {"encode": {"us": round(enc_s, 1), "GBps": round(n * L / enc_s / 1e3, 1),
+                          "frac_hbm": round(n * L / enc_s / 1e3 / HBM_PEAK_GBPS, 4)}}
+        for name, pos in patterns(n, k, rng).items():
+            pos = np.asarray(pos, np.uint8)
+            cw.copy_(good)
+            cw.view(n, L)[torch.from_numpy(pos.astype(np.int64)).cuda()] = 0
+            torch.cuda.synchronize()
+            ctx.recover(n, k, cw.data_ptr(), L, pos, stream)
+            torch.cuda.synchronize()
+            if not torch.equal(cw, good):
+                raise RuntimeError(f"C5 ({n},{k}) {name}: recover differs")
+            r_s, _ = _per_call(torch, stream, lambda: ctx.recover(n, k, cw.data_ptr(), L, pos, stream),
+                               budget_s=0.05, sync_reps=3)
+            b = (n + len(pos)) * L
+            res[f"recover_{name}"] = {"e": len(pos), "us": round(r_s, 1), "GBps": round(b / r_s / 1e3, 1),
+                                      "frac_hbm": round(b / r_s / 1e3 / HBM_PEAK_GBPS, 4)}
+        shapes[f"({n},{k})"] = res
+        del cw, good
+    # 65536 tiny codewords: (32,16), 64 B shards
+    count, n, k, Ls = 65536, 32, 16, 64
+    T = n - k
+    pool = torch.randint(0, 256, (count * n * Ls,), dtype=torch.uint8, device="cuda")
+    enc, rec, alg_e, alg_r = [], [], 0, 0
+    for i in range(count):
+        o = i * n * Ls
+        enc.append({"op": "encode", "n": n, "k": k, "shard_len": Ls, "message": pool[o + T * Ls:o + n * Ls],
+                    "parity": pool[o:o + T * Ls]})
+        pos = rng.permutation(n)[:int(rng.integers(1, T + 1))]
+        rec.append({"op": "recover", "n": n, "k": k, "shard_len": Ls, "codeword": pool[o:o + n * Ls],
+                    "positions": pos})
+        alg_e += n * Ls
+        alg_r += (n + len(pos)) * Ls
+    earr, ekeep = ars.Batch.make_jobs(enc)
+    rarr, rkeep = ars.Batch.make_jobs(rec)
+    b = ars.Batch(count)
+    tiny = {}
+    for name, arr, alg in (("encode", earr, alg_e), ("recover", rarr, alg_r)):
+        bt, _ = _per_call(torch, stream, lambda: b.run_jobs(arr, count, stream), budget_s=0.05, sync_reps=3)
+        pl, _ = _per_call(torch, stream, lambda: ars.lib().ars_run_jobs_plain(ctx.handle, arr, count,
+                                                                            stream.cuda_stream),
+                          budget_s=0.0, sync_reps=1, min_reps=1)
+        tiny[name] = {"batched_ms": round(bt / 1e3, 3), "plain_ms": round(pl / 1e3, 3),
+                      "batched_GBps": round(alg / bt / 1e3, 1), "plain_GBps": round(alg / pl / 1e3, 1),
+                      "speedup": round(pl / bt, 1)}
+    b.close()
+    del pool
+    return {"workload": "(256,224) and (256,16) at 4 MiB shards with the four erasure patterns; 65536 x (32,16) "
+                        "at 64 B shards, e~U{1..16}, one batch vs one call per codeword",
+            "shapes": shapes, "tiny_32x16_64B": tiny}
+
+
 def bench_host(ars, torch, np, n, k, L, e, reps=5, seed=46):
     """The host-slice entry points (the Rust binding's path): the same
     (n, k, L) codeword in pageable host memory, streamed through the GPU in
@@ -601,6 +815,8 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--no-decode", action="store_true")
+    ap.add_argument("--no-grid", action="store_true")
+    ap.add_argument("--no-c5", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every figure is "
@@ -696,7 +912,8 @@ def main():
     rec_gbps = (n + e) * L / s_rec / 1e9
     # the tile variants the autotuner kept (quads per workgroup, LDS buffers)
     enc_name = f"k_encode_tile<7, {tune_enc['quads']}, true, false, {tune_enc['buffers']}, true>"
-    rec_name = f"k_recover_tile<8, {tune_rec['quads']}, true, {tune_rec['buffers']}>"
+    rec_name = (f"k_recover_stream<8, {tune_rec['quads']}, true>" if tune_rec["buffers"] == 1
+                else f"k_recover_tile<8, {tune_rec['quads']}, true, {tune_rec['buffers']}>")
     if s_rec >= s_enc:
         dom, dom_bytes, dom_s = rec_name, (n + e) * L, s_rec
     else:
@@ -749,6 +966,12 @@ def main():
     if not args.no_decode:
         out["decode"] = bench_decode(ars, torch, np, n, k, L)
         print(f"decode: {out['decode']}", file=sys.stderr, flush=True)
+    if not args.no_grid:
+        out["reference_grid"] = bench_reference_grid(ars, torch, np, stream, with_cpu=not args.no_cpu)
+        print(f"reference grid: {out['reference_grid']['median_cpu_over_gpu']}", file=sys.stderr, flush=True)
+    if not args.no_c5:
+        out["c5"] = bench_c5(ars, torch, np, stream)
+        print(f"c5: {out['c5']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, k, L, e, args.cpu_seconds)
     print(json.dumps(out), flush=True)

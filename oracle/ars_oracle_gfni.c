@@ -84,28 +84,99 @@ TGT static void bfly_inv(uint8_t *a, uint8_t *b, size_t len, __m512i mat, int ze
     }
 }
 
-/* rs/src/kernel/gfni_kernel.rs:78-107 */
-TGT void aro_gfni_fft_sharded(uint8_t *shards, size_t L, int k, uint8_t beta) {
+/* rs/src/kernel/gfni_kernel.rs:78-107 (the generic recursion) */
+TGT static void fft_rec(uint8_t *shards, size_t L, int k, uint8_t beta) {
     if (k == 0) return;
-    g_init();
     size_t half = (size_t)1 << (k - 1);
     uint8_t tw = G_SPV[k - 1][beta];
     __m512i mat = _mm512_set1_epi64((long long)G_GFNI[tw]);
     for (size_t i = 0; i < half; i++) bfly_fwd(shards + i * L, shards + (i + half) * L, L, mat, tw == 0);
-    aro_gfni_fft_sharded(shards, L, k - 1, beta);
-    aro_gfni_fft_sharded(shards + half * L, L, k - 1, beta ^ G_BASIS[k - 1]);
+    fft_rec(shards, L, k - 1, beta);
+    fft_rec(shards + half * L, L, k - 1, beta ^ G_BASIS[k - 1]);
 }
 
 /* rs/src/kernel/gfni_kernel.rs:109-150 */
-TGT void aro_gfni_ifft_sharded(uint8_t *shards, size_t L, int k, uint8_t beta) {
+TGT static void ifft_rec(uint8_t *shards, size_t L, int k, uint8_t beta) {
     if (k == 0) return;
-    g_init();
     size_t half = (size_t)1 << (k - 1);
-    aro_gfni_ifft_sharded(shards, L, k - 1, beta);
-    aro_gfni_ifft_sharded(shards + half * L, L, k - 1, beta ^ G_BASIS[k - 1]);
+    ifft_rec(shards, L, k - 1, beta);
+    ifft_rec(shards + half * L, L, k - 1, beta ^ G_BASIS[k - 1]);
     uint8_t tw = G_SPV[k - 1][beta];
     __m512i mat = _mm512_set1_epi64((long long)G_GFNI[tw]);
     for (size_t i = 0; i < half; i++) bfly_inv(shards + i * L, shards + (i + half) * L, L, mat, tw == 0);
+}
+
+/* The reference's generated kernels for beta = 0 (rs/build.rs:386-436,
+ * 474-527; dispatched at rs/src/kernel/gfni_kernel.rs:200-240): the same
+ * butterflies in the same order as the recursion -- preorder for the FFT,
+ * postorder for the IFFT -- as one straight sequence with each butterfly's
+ * matrix fixed in advance (there, compile-time constants).  Here the
+ * sequence for each size is built once: (first shard, half width, matrix,
+ * zero twiddle). */
+typedef struct {
+    uint32_t off, half;
+    uint64_t mat;
+    int zero;
+} B0Step;
+static B0Step B0_FFT[9][255], B0_IFFT[9][255];
+static int B0_LEN[9], b0_ready = 0, g_unrolled = 1;
+
+static void b0_build(B0Step *seq, int *len, int k, uint8_t beta, uint32_t off, int pre) {
+    if (k == 0) return;
+    uint32_t half = 1u << (k - 1);
+    uint8_t tw = G_SPV[k - 1][beta];
+    B0Step st = {off, half, G_GFNI[tw], tw == 0};
+    if (pre) seq[(*len)++] = st;
+    b0_build(seq, len, k - 1, beta, off, pre);
+    b0_build(seq, len, k - 1, beta ^ G_BASIS[k - 1], off + half, pre);
+    if (!pre) seq[(*len)++] = st;
+}
+
+static void b0_init(void) {
+    if (b0_ready) return;
+    for (int k = 1; k <= 8; k++) {
+        int a = 0, b = 0;
+        b0_build(B0_FFT[k], &a, k, 0, 0, 1);
+        b0_build(B0_IFFT[k], &b, k, 0, 0, 0);
+        B0_LEN[k] = a;
+    }
+    b0_ready = 1;
+}
+
+/* 1 (default): beta = 0 transforms take the fixed sequences above, as the
+ * reference's GfniKernel dispatches them; 0: the generic recursion. */
+void aro_gfni_set_unrolled(int on) { g_unrolled = on != 0; }
+
+TGT void aro_gfni_fft_sharded(uint8_t *shards, size_t L, int k, uint8_t beta) {
+    g_init();
+    if (k == 0) return;
+    if (beta == 0 && g_unrolled && k <= 8) {
+        b0_init();
+        for (int i = 0; i < B0_LEN[k]; i++) {
+            const B0Step st = B0_FFT[k][i];
+            __m512i mat = _mm512_set1_epi64((long long)st.mat);
+            uint8_t *blk = shards + (size_t)st.off * L;
+            for (size_t j = 0; j < st.half; j++) bfly_fwd(blk + j * L, blk + (j + st.half) * L, L, mat, st.zero);
+        }
+        return;
+    }
+    fft_rec(shards, L, k, beta);
+}
+
+TGT void aro_gfni_ifft_sharded(uint8_t *shards, size_t L, int k, uint8_t beta) {
+    g_init();
+    if (k == 0) return;
+    if (beta == 0 && g_unrolled && k <= 8) {
+        b0_init();
+        for (int i = 0; i < B0_LEN[k]; i++) {
+            const B0Step st = B0_IFFT[k][i];
+            __m512i mat = _mm512_set1_epi64((long long)st.mat);
+            uint8_t *blk = shards + (size_t)st.off * L;
+            for (size_t j = 0; j < st.half; j++) bfly_inv(blk + j * L, blk + (j + st.half) * L, L, mat, st.zero);
+        }
+        return;
+    }
+    ifft_rec(shards, L, k, beta);
 }
 
 /* rs/src/kernel/gfni_kernel.rs:152-173 */
@@ -214,11 +285,17 @@ static int cmp_d(const void *a, const void *b) {
     return (x > y) - (x < y);
 }
 
-int aro_gfni_time_codec(size_t n, size_t T, size_t L, const uint8_t *pos, size_t e, int reps, double *out_ns) {
-    if (!aro_gfni_supported() || reps < 1 || n < 2 || n > 256 || T >= n || e > T) return -1;
+int aro_gfni_time_codec(size_t n, size_t T, size_t L, const uint8_t *pos, size_t e, int reps, size_t offset,
+                        double *out_ns) {
+    if (!aro_gfni_supported() || reps < 1 || n < 2 || n > 256 || T >= n || e > T || offset >= 64) return -1;
     g_init();
     size_t k = n - T;
-    uint8_t *msg = malloc(k * L), *par = malloc(T * L), *rx = malloc(n * L), *ws = malloc(n * L);
+    /* message, parity and codeword start `offset` bytes past a 64-byte
+     * boundary (the reference's benches time offsets 0 and 1,
+     * rs/benches/encode_systematic_sharded.rs:51-73) */
+    uint8_t *mb = aligned_alloc(64, k * L + 128), *pb = aligned_alloc(64, T * L + 128),
+            *rb = aligned_alloc(64, n * L + 128), *ws = aligned_alloc(64, n * L + 64);
+    uint8_t *msg = mb ? mb + offset : NULL, *par = pb ? pb + offset : NULL, *rx = rb ? rb + offset : NULL;
     double *te = malloc(sizeof(double) * reps), *tr = malloc(sizeof(double) * reps);
     int rc = -1;
     if (!msg || !par || !rx || !ws || !te || !tr) goto done;
@@ -249,6 +326,6 @@ int aro_gfni_time_codec(size_t n, size_t T, size_t L, const uint8_t *pos, size_t
     out_ns[1] = tr[reps / 2];
     rc = 0;
 done:
-    free(msg); free(par); free(rx); free(ws); free(te); free(tr);
+    free(mb); free(pb); free(rb); free(ws); free(te); free(tr);
     return rc;
 }

@@ -62,8 +62,10 @@ def lib():
         L.aro_gfni_supported.restype = ctypes.c_int
         L.aro_gfni_supported.argtypes = []
         L.aro_gfni_time_codec.restype = ctypes.c_int
-        L.aro_gfni_time_codec.argtypes = [_sz, _sz, _sz, _u8p, _sz, ctypes.c_int,
+        L.aro_gfni_time_codec.argtypes = [_sz, _sz, _sz, _u8p, _sz, ctypes.c_int, _sz,
                                           ctypes.POINTER(ctypes.c_double)]
+        L.aro_gfni_set_unrolled.restype = None
+        L.aro_gfni_set_unrolled.argtypes = [ctypes.c_int]
         L.aro_decode_systematic_scalar.restype = ctypes.c_int
         L.aro_decode_systematic_scalar.argtypes = [_sz, _sz, _u8p, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         L.aro_decode_sharded.restype = ctypes.c_int
@@ -283,10 +285,17 @@ def decode_sharded(n, T, received, L, formal=False):
     return rc, status
 
 
-def gfni_time_codec(n, T, L, positions, reps):
+def gfni_time_codec(n, T, L, positions, reps, offset=0):
     """Median ns of one GFNI encode and one recover, timed inside C on
-    buffers allocated once (CPU baseline only).  None without GFNI."""
+    buffers allocated once, starting `offset` bytes past a 64-byte boundary
+    (CPU baseline only).  None without GFNI."""
     pos = np.ascontiguousarray(np.asarray(positions, dtype=np.uint8))
     out = (ctypes.c_double * 2)()
-    rc = lib().aro_gfni_time_codec(n, T, L, _p(pos), len(pos), reps, out)
+    rc = lib().aro_gfni_time_codec(n, T, L, _p(pos), len(pos), reps, offset, out)
     return None if rc else (out[0], out[1])
+
+
+def gfni_set_unrolled(on):
+    """beta = 0 GFNI transforms as the reference's generated fixed sequences
+    (True, its default dispatch) or through the generic recursion (False)."""
+    lib().aro_gfni_set_unrolled(1 if on else 0)

@@ -176,8 +176,10 @@ def unguard(buf, shift, size, pad=256):
 def test_guarded_buffers_match_oracle(ars, n, T, shift):
     """Shard length 68 (17 column quads: the last tile is partial) on buffers
     that start `shift` bytes past a 4-byte boundary.  Shift 0 takes the dword
-    I/O path, shifts 1-3 the byte-wise one; results and the guard bytes on
-    both sides of every buffer are exact."""
+    I/O path; recover at shifts 1-3 the shifted dword path (one buffer, one
+    shift); encode at shifts 1 and 3 the byte-wise path (message and parity
+    at different shifts) and at shift 2 the shifted dword path; results and
+    the guard bytes on both sides of every buffer are exact."""
     k, L = n - T, 68
     rng = np.random.default_rng(n * 7 + T * 3 + shift)
     ctx = ars.Context(0)
@@ -208,6 +210,43 @@ def test_guarded_buffers_match_oracle(ars, n, T, shift):
         b, p = guarded(lost, shift)
         ctx.recompute_data(n, k, p, L)
         assert np.array_equal(unguard(b, shift, n * L), cw)
+    ctx.close()
+
+
+@pytest.mark.parametrize("shift", [1, 2, 3])
+@pytest.mark.parametrize("n,T", [(256, 128), (64, 32)])
+def test_shifted_dword_path_equals_aligned(ars, n, T, shift):
+    """Message and parity (and the codeword) both `shift` bytes past a 4-byte
+    boundary with shard_len % 4 == 0: the shifted dword path (each lane's
+    dword spans the boundary between columns) on launches of many tiles,
+    against the same code on aligned buffers, and no byte outside the
+    buffers written."""
+    k, L = n - T, 65536 + 68
+    rng = np.random.default_rng(n + T + shift)
+    ctx = ars.Context(0)
+    msg = rng.integers(0, 256, k * L, dtype=np.uint8)
+    bm0, pm0 = guarded(msg, 0)
+    bp0, pp0 = guarded(np.zeros(T * L, np.uint8), 0)
+    ctx.encode(n, k, pm0, pp0, L)
+    exp = unguard(bp0, 0, T * L)
+    # the aligned result, pinned on the first and last 8 columns by the oracle
+    for cols in (slice(0, 8), slice(L - 8, L)):
+        assert np.array_equal(exp.reshape(T, L)[:, cols],
+                              O.encode(n, T, msg.reshape(k, L)[:, cols].copy().reshape(-1), 8).reshape(T, 8))
+    bm, pm = guarded(msg, shift)
+    bp, pp = guarded(np.zeros(T * L, np.uint8), shift)
+    ctx.encode(n, k, pm, pp, L)
+    assert np.array_equal(unguard(bp, shift, T * L), exp)
+    assert np.array_equal(unguard(bm, shift, k * L), msg)
+    cw = np.concatenate([exp, msg])
+    pos = rng.permutation(n)[:T].astype(np.uint8)
+    rx = cw.copy().reshape(n, L)
+    rx[pos] = rng.integers(0, 256, (T, L), dtype=np.uint8)  # garbage: the result is c ^ g
+    want = cw.copy().reshape(n, L)
+    want[pos] ^= rx[pos]
+    b, p = guarded(rx.reshape(-1), shift)
+    assert ctx.recover(n, k, p, L, pos) == 1
+    assert np.array_equal(unguard(b, shift, n * L), want.reshape(-1))
     ctx.close()
 
 

@@ -84,12 +84,14 @@ def test_default_kernels_do_not_spill(asm):
     # points, PQ 64 / NBUF 2 below; the one-launch recovers use the same variants.
     # A spill matters doubly in the streaming kernel: a scratch reload waits for
     # every memory operation issued before it, the prefetched rows included.
-    want = [r"k_encode_tileILi7ELi32ELb1ELb0ELi1ELb1E", r"k_encode_tileILi6ELi64ELb1ELb0ELi2E",
-            r"k_recover_streamILi8ELi64ELb1E", r"k_recover_streamILi8ELi64ELb0E",
-            r"k_recover_streamILi7ELi64ELb1E", r"k_recover_streamILi5ELi64ELb1E",
-            r"k_recover_stream_fusedILi8ELi64ELb1E", r"k_recover_tileILi8ELi32ELb0ELi2E",
+    # (encode and the streaming recover in access modes 1 and 2: aligned dwords
+    # and dwords shifted past alignment)
+    want = [r"k_encode_tileILi7ELi32ELi[12]ELb0ELi1ELb1E", r"k_encode_tileILi6ELi64ELi[12]ELb0ELi2E",
+            r"k_recover_streamILi8ELi64ELb1ELi[12]E", r"k_recover_streamILi8ELi64ELb0ELi[12]E",
+            r"k_recover_streamILi7ELi64ELb1ELi[12]E", r"k_recover_streamILi5ELi64ELb1ELi[12]E",
+            r"k_recover_stream_fusedILi8ELi64ELb1ELi[12]E", r"k_recover_tileILi8ELi32ELb0ELi2E",
             r"k_recover_tile_fusedILi8ELi32ELb0ELi2E", r"k_recover_tile_fusedILi5ELi64ELb0ELi2E",
-            r"k_recover_small_fusedILi4ELb0E", r"k_syndrome_tile", r"k_decode_columns"]
+            r"k_recover_small_fusedILi4ELi[012]E", r"k_syndrome_tile", r"k_decode_columns"]
     seen = {}
     unmatched = []
     for w in want:

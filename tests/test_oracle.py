@@ -160,20 +160,26 @@ def test_sharded_equals_columnwise():
 
 
 @pytest.mark.skipif(not O.gfni_supported(), reason="host lacks AVX-512BW+GFNI")
+@pytest.mark.parametrize("unrolled", [True, False])
 @pytest.mark.parametrize("L", [1, 63, 64, 65, 128, 200])
-def test_gfni_transforms_equal_lut(L):
-    # rs/src/kernel/gfni_kernel.rs:324-387
-    for k in range(1, 6):
-        n = 1 << k
-        sh = np.array([(i * 37 + j * 13 + 1) & 0xFF for i in range(n) for j in range(L)], np.uint8)
-        for beta in (0, SUB[n % 256]):
-            a, b = sh.copy(), sh.copy()
-            O.fft_sharded(a, L, k, beta)
-            O.fft_sharded(b, L, k, beta, gfni=True)
-            assert np.array_equal(a, b)
-            O.ifft_sharded(a, L, k, beta)
-            O.ifft_sharded(b, L, k, beta, gfni=True)
-            assert np.array_equal(a, b) and np.array_equal(a, sh)
+def test_gfni_transforms_equal_lut(L, unrolled):
+    # rs/src/kernel/gfni_kernel.rs:324-387; beta = 0 through the generated
+    # fixed sequences (the reference's dispatch) and through the recursion
+    O.gfni_set_unrolled(unrolled)
+    try:
+        for k in range(1, 9):
+            n = 1 << k
+            sh = np.array([(i * 37 + j * 13 + 1) & 0xFF for i in range(n) for j in range(L)], np.uint8)
+            for beta in (0, SUB[n % 256]):
+                a, b = sh.copy(), sh.copy()
+                O.fft_sharded(a, L, k, beta)
+                O.fft_sharded(b, L, k, beta, gfni=True)
+                assert np.array_equal(a, b), (k, beta)
+                O.ifft_sharded(a, L, k, beta)
+                O.ifft_sharded(b, L, k, beta, gfni=True)
+                assert np.array_equal(a, b) and np.array_equal(a, sh), (k, beta)
+    finally:
+        O.gfni_set_unrolled(True)
 
 
 # ------------------------------------------------------------------ encode
