@@ -48,12 +48,13 @@ int hip_fail(hipError_t e, const char *where) { return ars::hip_error(e, where);
 
 }  // namespace
 
-int ars::check_code(uint32_t n, uint32_t k, size_t L) {
+int ars::check_code(uint32_t n, uint32_t k, size_t L, bool wide) {
     if (n < 2 || n > 256 || !is_pow2(n)) return fail(ARS_E_INVAL, "n must be a power of two in [2, 256]");
     if (k < 1 || k >= n) return fail(ARS_E_INVAL, "k must satisfy 1 <= k < n");
     if (L == 0) return fail(ARS_E_INVAL, "shard_len must be >= 1");
-    if ((uint64_t)n * L > 0xFFFFFFFFull)
-        return fail(ARS_E_INVAL, "codeword larger than 4 GiB (n * shard_len must fit 32 bits)");
+    if (L > 0xFFFFFFFFull) return fail(ARS_E_INVAL, "shard_len must be below 4 GiB");
+    if (!wide && (uint64_t)n * L > 0xFFFFFFFFull)
+        return fail(ARS_E_INVAL, "codeword of 4 GiB or more: this call needs n * shard_len below 4 GiB");
     const uint32_t T = n - k;
     if (!is_pow2(T) && !is_pow2(k))
         return fail(ARS_E_UNSUPPORTED, "neither T = n - k nor k is a power of two");
@@ -62,7 +63,16 @@ int ars::check_code(uint32_t n, uint32_t k, size_t L) {
 
 namespace {
 using ars::check_code;
-bool aligned4(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 3u) == 0; }
+
+// Access mode of a call on buffers a and b (ars::io_mode).  Codewords of 4 GiB
+// or more run only as aligned dwords (mode 3).
+int call_mode(const void *a, const void *b, uint32_t n, size_t L, int &am) {
+    am = ars::io_mode(a, b, L, (uint64_t)n * L);
+    if (am < 0)
+        return fail(ARS_E_UNSUPPORTED,
+                    "a codeword of 4 GiB or more needs 4-byte aligned buffers and shard_len % 4 == 0");
+    return ARS_OK;
+}
 
 // Scratch of ars_recover_noalloc: room for one erasure plan (n <= 256).  Every
 // GPU the library runs on holds its own copy until the process ends.  Host
@@ -158,15 +168,15 @@ int ars_destroy(ars_ctx *ctx) {
 int ars_encode(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *d_message, uint8_t *d_parity,
                size_t L, uint32_t flags, void *stream) {
     if (!ctx || !d_message || !d_parity) return fail(ARS_E_NULL, "null context or buffer");
-    int rc = check_code(n, k, L);
+    int rc = check_code(n, k, L, true);
     if (rc) return rc;
     if (flags & ~ARS_ENC_ACCUMULATE) return fail(ARS_E_INVAL, "unknown encode flags");
+    int am;
+    if ((rc = call_mode(d_message, d_parity, n, L, am))) return rc;
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const uint32_t T = n - k;
-    const int am = ars::io_mode(d_message, d_parity, L);
-    const bool al = am == 1;
     if (is_pow2(T)) {
         const int tl = ilog2(T);
         e = ars::launch_encode_ref(tl, am, (flags & ARS_ENC_ACCUMULATE) != 0, d_message, d_parity,
@@ -174,7 +184,7 @@ int ars_encode(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *d_message, u
     } else {
         if (flags & ARS_ENC_ACCUMULATE)
             return fail(ARS_E_UNSUPPORTED, "ACCUMULATE is defined only when T is a power of two");
-        e = ars::launch_encode_kpow2(ilog2(k), al, d_message, d_parity, L, n / k - 1, n - k,
+        e = ars::launch_encode_kpow2(ilog2(k), am, d_message, d_parity, L, n / k - 1, n - k,
                                      ctx->d_tw, s);
     }
     if (e != hipSuccess) return hip_fail(e, "encode launch");
@@ -217,11 +227,12 @@ int ars_recover_plan(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *positi
 int ars_recover_with_plan(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_t L,
                           const void *d_plan, void *stream) {
     if (!ctx || !d_received || !d_plan) return fail(ARS_E_NULL, "null context, buffer or plan");
-    int rc = check_code(n, k, L);
+    int rc = check_code(n, k, L, true);
     if (rc) return rc;
+    int am;
+    if ((rc = call_mode(d_received, d_received, n, L, am))) return rc;
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    const int am = ars::io_mode(d_received, d_received, L);
     const int nl = ilog2(n);
     e = ars::launch_recover(nl, am, d_received, L, static_cast<const ars::RecPrep *>(d_plan),
                             ctx->d_tw, static_cast<hipStream_t>(stream), ctx->tune_pq[1][nl], ctx->tune_nbuf[1][nl],
@@ -233,8 +244,10 @@ int ars_recover_with_plan(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_recei
 int ars_recover(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_t L,
                 const uint8_t *positions, uint32_t e_cnt, void *stream) {
     if (!ctx || !d_received) return fail(ARS_E_NULL, "null context or buffer");
-    int rc = check_code(n, k, L);
+    int rc = check_code(n, k, L, true);
     if (rc) return rc;
+    int am;
+    if ((rc = call_mode(d_received, d_received, n, L, am))) return rc;
     const uint32_t T = n - k;
     // rs/src/codec.rs:491-496: too many erasures -> false, none -> true.
     if (e_cnt > T) return ARS_FALSE;
@@ -244,7 +257,6 @@ int ars_recover(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const int am = ars::io_mode(d_received, d_received, L);
     const int nl = ilog2(n);
     // Launches of at most one workgroup per compute unit compute the plan in
     // the recover kernel itself: one launch instead of two.
@@ -277,8 +289,10 @@ int ars_recover(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_
 int ars_recover_noalloc(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_t L,
                         const uint8_t *positions, uint32_t e_cnt, void *stream) {
     if (!ctx || !d_received) return fail(ARS_E_NULL, "null context or buffer");
-    int rc = check_code(n, k, L);  // n <= 256: the plan fits the scratch (static_assert above)
+    int rc = check_code(n, k, L, true);  // n <= 256: the plan fits the scratch (static_assert above)
     if (rc) return rc;
+    int am;
+    if ((rc = call_mode(d_received, d_received, n, L, am))) return rc;
     const uint32_t T = n - k;
     if (e_cnt > T) return ARS_FALSE;
     if (e_cnt == 0) return ARS_TRUE;
@@ -288,7 +302,6 @@ int ars_recover_noalloc(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_receive
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const int am = ars::io_mode(d_received, d_received, L);
     const int nl = ilog2(n);
     e = ars::launch_recover_fused(nl, am, d_received, L, pa, e_cnt, ctx->d_gtab, ctx->d_tw, s,
                                   (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256));
@@ -304,13 +317,15 @@ int ars_recover_noalloc(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_receive
 int ars_recompute_data(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *d_received, size_t L,
                        void *stream) {
     if (!ctx || !d_received) return fail(ARS_E_NULL, "null context or buffer");
-    int rc = check_code(n, k, L);
+    int rc = check_code(n, k, L, true);
     if (rc) return rc;
     if (n != 2 * (n - k)) return fail(ARS_E_INVAL, "recompute_data requires n = 2T");
+    int am;
+    if ((rc = call_mode(d_received, d_received, n, L, am))) return rc;
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    const bool al = (L % 4 == 0) && aligned4(d_received);
-    e = ars::launch_recompute(ilog2(n - k), al, d_received, L, ctx->d_tw, static_cast<hipStream_t>(stream));
+    // mode 2 runs as bytes here (launch_recompute)
+    e = ars::launch_recompute(ilog2(n - k), am, d_received, L, ctx->d_tw, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "recompute launch");
     return ARS_OK;
 }

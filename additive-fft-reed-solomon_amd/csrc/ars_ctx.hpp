@@ -44,7 +44,9 @@ struct ars_ctx {
 namespace ars {
 int set_error(int code, const std::string &msg);
 int hip_error(hipError_t e, const char *where);
-int check_code(uint32_t n, uint32_t k, size_t L);
+// wide: the call runs codewords of 4 GiB or more (n * shard_len >= 2^32);
+// otherwise those are rejected.  shard_len itself stays below 4 GiB.
+int check_code(uint32_t n, uint32_t k, size_t L, bool wide = false);
 bool is_pow2(uint32_t v);
 int ilog2(uint32_t v);
 void release_host_pipe(ars_ctx *ctx);     // ars_host.cpp

@@ -200,7 +200,8 @@ extern "C" {
 int ars_encode_host(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *message, uint8_t *parity,
                     size_t L, uint32_t flags) {
     if (!ctx || !message || !parity) return ars::set_error(ARS_E_NULL, "null context or buffer");
-    int rc = ars::check_code(n, k, L);
+    // any codeword size: the device sees column chunks of at most kChunkBytes
+    int rc = ars::check_code(n, k, L, true);
     if (rc) return rc;
     if (flags & ~ARS_ENC_ACCUMULATE) return ars::set_error(ARS_E_INVAL, "unknown encode flags");
     const uint32_t T = n - k;
@@ -252,7 +253,7 @@ int ars_encode_host(ars_ctx *ctx, uint32_t n, uint32_t k, const uint8_t *message
 int ars_recover_host(ars_ctx *ctx, uint32_t n, uint32_t k, uint8_t *received, size_t L,
                      const uint8_t *positions, uint32_t e_cnt) {
     if (!ctx || !received) return ars::set_error(ARS_E_NULL, "null context or buffer");
-    int rc = ars::check_code(n, k, L);
+    int rc = ars::check_code(n, k, L, true);  // column chunks, as in ars_encode_host
     if (rc) return rc;
     if (e_cnt > n - k) return ARS_FALSE;
     if (e_cnt == 0) return ARS_TRUE;

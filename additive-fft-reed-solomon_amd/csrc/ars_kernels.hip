@@ -87,7 +87,7 @@ __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
     body_encode_tile<TL, PQ, ACC, NBUF, BufIO<AM>, true>(io, q, p, tile, ONE ? 1u : chunks, tw, x0, g_tw);
 }
 
-template <int KL, bool AL>
+template <int KL, int AM>
 __global__ void __launch_bounds__(256) k_encode_kpow2_small(const uint8_t *__restrict__ msg,
                                                             uint8_t *__restrict__ par, uint32_t L,
                                                             uint32_t blocks, uint32_t msg_base,
@@ -96,11 +96,11 @@ __global__ void __launch_bounds__(256) k_encode_kpow2_small(const uint8_t *__res
     __shared__ MulTab tw[kTwiddleEntries];
     stage_twiddles<256>(tw, g_tw);
     __syncthreads();
-    BufIO<AL> io = make_bufio<AL>(msg, K * L, par, blocks * K * L, blockIdx.x * blockDim.x + threadIdx.x, L);
+    BufIO<AM> io = make_bufio<AM>(msg, K * L, par, blocks * K * L, blockIdx.x * blockDim.x + threadIdx.x, L);
     body_encode_kpow2_small<KL>(io, blocks, msg_base, tw);
 }
 
-template <int KL, int PQ, bool AL>
+template <int KL, int PQ, int AM>
 __global__ void __launch_bounds__((Tile<KL, PQ>::THREADS))
     k_encode_kpow2_tile(const uint8_t *__restrict__ msg, uint8_t *__restrict__ par, uint32_t L,
                         uint32_t blocks, uint32_t msg_base, const MulTab *__restrict__ g_tw) {
@@ -110,7 +110,7 @@ __global__ void __launch_bounds__((Tile<KL, PQ>::THREADS))
     stage_twiddles<TT::THREADS>(tw, g_tw);
     __syncthreads();
     const int p = threadIdx.x % PQ, q = tile_q<PQ>();
-    BufIO<AL> io = make_bufio<AL>(msg, TT::N * L, par, blocks * TT::N * L, blockIdx.x * PQ + p, L);
+    BufIO<AM> io = make_bufio<AM>(msg, TT::N * L, par, blocks * TT::N * L, blockIdx.x * PQ + p, L);
     body_encode_kpow2_tile<KL, PQ, 2>(io, q, p, tile, blocks, msg_base, tw);
 }
 
@@ -276,18 +276,18 @@ __global__ void __launch_bounds__((Tile<NL, PQ>::THREADS)) __attribute__((amdgpu
     recover_stream<NL, PQ, OOB, AM>(data, L, tiles, s_tab, s_er, tile, tw);
 }
 
-template <int TL, bool AL>
+template <int TL, int AM>
 __global__ void __launch_bounds__(256) k_recompute_small(uint8_t *__restrict__ data, uint32_t L,
                                                          const MulTab *__restrict__ g_tw) {
     constexpr int T = 1 << TL;
     __shared__ MulTab tw[kTwiddleEntries];
     stage_twiddles<256>(tw, g_tw);
     __syncthreads();
-    BufIO<AL> io = make_bufio<AL>(data, 2 * T * L, data, 2 * T * L, blockIdx.x * blockDim.x + threadIdx.x, L);
+    BufIO<AM> io = make_bufio<AM>(data, 2 * T * L, data, 2 * T * L, blockIdx.x * blockDim.x + threadIdx.x, L);
     body_recompute_small<TL>(io, tw);
 }
 
-template <int TL, int PQ, bool AL>
+template <int TL, int PQ, int AM>
 __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
     k_recompute_tile(uint8_t *__restrict__ data, uint32_t L, const MulTab *__restrict__ g_tw) {
     using TT = Tile<TL, PQ>;
@@ -296,7 +296,7 @@ __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
     stage_twiddles<TT::THREADS>(tw, g_tw);
     __syncthreads();
     const int p = threadIdx.x % PQ, q = tile_q<PQ>();
-    BufIO<AL> io = make_bufio<AL>(data, 2 * TT::N * L, data, 2 * TT::N * L, blockIdx.x * PQ + p, L);
+    BufIO<AM> io = make_bufio<AM>(data, 2 * TT::N * L, data, 2 * TT::N * L, blockIdx.x * PQ + p, L);
     body_recompute_tile<TL, PQ>(io, q, p, tile, tw);
 }
 
@@ -341,7 +341,11 @@ static hipError_t enc_one(const uint8_t *msg, uint8_t *par, uint32_t L, uint32_t
                                dim3(Tile<TL, PQ_>::THREADS), 0, s, msg, par, L, chunks, tw);         \
         return hipGetLastError();                                                                    \
     }
-        ENC_V(16, 1) ENC_V(16, 2) ENC_V(32, 1) ENC_V(32, 2) ENC_V(64, 1) ENC_V(64, 2)
+        if constexpr (AM == 3) {
+            ENC_V(64, 1) ENC_V(64, 2)
+        } else {
+            ENC_V(16, 1) ENC_V(16, 2) ENC_V(32, 1) ENC_V(32, 2) ENC_V(64, 1) ENC_V(64, 2)
+        }
 #undef ENC_V
         return hipErrorInvalidValue;
     }
@@ -356,6 +360,8 @@ static hipError_t enc_acc(int am, bool acc, const uint8_t *msg, uint8_t *par, ui
                            : enc_one<TL, 1, false>(msg, par, L, chunks, tw, s, pq, nbuf);
         case 2: return acc ? enc_one<TL, 2, true>(msg, par, L, chunks, tw, s, pq, nbuf)
                            : enc_one<TL, 2, false>(msg, par, L, chunks, tw, s, pq, nbuf);
+        case 3: return acc ? enc_one<TL, 3, true>(msg, par, L, chunks, tw, s, pq, nbuf)
+                           : enc_one<TL, 3, false>(msg, par, L, chunks, tw, s, pq, nbuf);
         default: return acc ? enc_one<TL, 0, true>(msg, par, L, chunks, tw, s, pq, nbuf)
                             : enc_one<TL, 0, false>(msg, par, L, chunks, tw, s, pq, nbuf);
     }
@@ -364,7 +370,7 @@ static hipError_t enc_acc(int am, bool acc, const uint8_t *msg, uint8_t *par, ui
 hipError_t launch_encode_ref(int tl, int am, bool acc, const uint8_t *msg, uint8_t *par,
                              uint32_t L, uint32_t chunks, const MulTab *tw, hipStream_t s,
                              int pq, int nbuf) {
-    if (pq == 0) pq = default_pq(0, tl);
+    if (pq == 0 || (am == 3 && tl > 4)) pq = am == 3 ? 64 : default_pq(0, tl);
     if (nbuf == 0) nbuf = default_nbuf(0, tl);
     if (!variant_ok(tl, pq, nbuf)) return hipErrorInvalidValue;
     switch (tl) {
@@ -380,26 +386,28 @@ hipError_t launch_encode_ref(int tl, int am, bool acc, const uint8_t *msg, uint8
     }
 }
 
-template <int KL, bool AL>
+template <int KL, int AM>
 static hipError_t kp_one(const uint8_t *msg, uint8_t *par, uint32_t L, uint32_t blocks,
                          uint32_t msg_base, const MulTab *tw, hipStream_t s) {
     const uint64_t quads = ((uint64_t)L + 3) / 4;
     if constexpr (KL <= 4) {
-        hipLaunchKernelGGL((k_encode_kpow2_small<KL, AL>), dim3(ceil_div(quads, 256)), dim3(256), 0, s,
+        hipLaunchKernelGGL((k_encode_kpow2_small<KL, AM>), dim3(ceil_div(quads, 256)), dim3(256), 0, s,
                            msg, par, L, blocks, msg_base, tw);
     } else {
         constexpr int PQ = tile_pq<KL>();
-        hipLaunchKernelGGL((k_encode_kpow2_tile<KL, PQ, AL>), dim3(ceil_div(quads, PQ)),
+        static_assert(AM != 3 || PQ % 64 == 0, "wide rows need one shard group per wave");
+        hipLaunchKernelGGL((k_encode_kpow2_tile<KL, PQ, AM>), dim3(ceil_div(quads, PQ)),
                            dim3(Tile<KL, PQ>::THREADS), 0, s, msg, par, L, blocks, msg_base, tw);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_encode_kpow2(int kl, bool al, const uint8_t *msg, uint8_t *par, uint32_t L,
+hipError_t launch_encode_kpow2(int kl, int am, const uint8_t *msg, uint8_t *par, uint32_t L,
                                uint32_t blocks, uint32_t msg_base, const MulTab *tw, hipStream_t s) {
-#define KP_CASE(K) \
-    case K: return al ? kp_one<K, true>(msg, par, L, blocks, msg_base, tw, s) \
-                      : kp_one<K, false>(msg, par, L, blocks, msg_base, tw, s);
+#define KP_CASE(K)                                                                \
+    case K: return am == 3 ? kp_one<K, 3>(msg, par, L, blocks, msg_base, tw, s)   \
+                 : am == 1 ? kp_one<K, 1>(msg, par, L, blocks, msg_base, tw, s)   \
+                           : kp_one<K, 0>(msg, par, L, blocks, msg_base, tw, s);
     switch (kl) {
         KP_CASE(0) KP_CASE(1) KP_CASE(2) KP_CASE(3) KP_CASE(4) KP_CASE(5) KP_CASE(6) KP_CASE(7)
         default: return hipErrorInvalidValue;
@@ -416,7 +424,10 @@ hipError_t launch_recover_prep(RecPrep *out, uint32_t n, uint32_t e, const PosAr
 // Erased rows may be read at the buffer's end (see recover_rows_in) when no
 // row offset added to it wraps past 2^32 (8 bytes of slack for access mode 2's
 // rounding).
-static bool oob_ok(int nl, uint32_t L) { return ((uint64_t)(1u << nl) + 15u) * L + 8u < (1ull << 32); }
+// Mode 3 reads them at the end of the row's own resource, which always works.
+static bool oob_ok(int nl, int am, uint32_t L) {
+    return am == 3 || ((uint64_t)(1u << nl) + 15u) * L + 8u < (1ull << 32);
+}
 
 // Workgroups of `kernel` resident per compute unit (at least 1).
 template <class K>
@@ -429,9 +440,9 @@ static uint32_t resident_groups(K kernel, int threads) {
 // The streaming recover's grid: twice the workgroups the device holds at
 // once (measured faster than exactly one round of resident workgroups), at
 // most one per tile.
-template <int NL, int PQ>
+template <int NL, int PQ, int AM>
 static uint32_t stream_grid(uint32_t tiles, uint32_t cus) {
-    static const uint32_t occ = resident_groups(k_recover_stream<NL, PQ, true, 1>, Tile<NL, PQ>::THREADS);
+    static const uint32_t occ = resident_groups(k_recover_stream<NL, PQ, true, AM>, Tile<NL, PQ>::THREADS);
     const uint64_t g = 2ull * occ * (cus ? cus : 256u);
     return (uint32_t)(g < tiles ? g : tiles);
 }
@@ -446,11 +457,11 @@ static hipError_t rec_one(uint8_t *data, uint32_t L, const RecPrep *prep, const 
         return hipGetLastError();
     } else {
         if constexpr (AM != 0) {
-            if (nbuf == 1 || AM == 2) {
+            if (nbuf == 1 || AM >= 2) {
 #define RS_V(PQ_)                                                                                    \
     if (pq == PQ_) {                                                                                 \
-        const uint32_t tiles = ceil_div(quads, PQ_), grid = stream_grid<NL, PQ_>(tiles, cus);        \
-        if (oob_ok(NL, L))                                                                           \
+        const uint32_t tiles = ceil_div(quads, PQ_), grid = stream_grid<NL, PQ_, AM>(tiles, cus);    \
+        if (oob_ok(NL, AM, L))                                                                       \
             hipLaunchKernelGGL((k_recover_stream<NL, PQ_, true, AM>), dim3(grid),                    \
                                dim3(Tile<NL, PQ_>::THREADS), 0, s, data, L, tiles, prep, tw);        \
         else                                                                                         \
@@ -458,12 +469,16 @@ static hipError_t rec_one(uint8_t *data, uint32_t L, const RecPrep *prep, const 
                                dim3(Tile<NL, PQ_>::THREADS), 0, s, data, L, tiles, prep, tw);        \
         return hipGetLastError();                                                                    \
     }
-                RS_V(16) RS_V(32) RS_V(64)
+                if constexpr (AM == 3) {
+                    RS_V(64)  // wide rows need one shard group per wave
+                } else {
+                    RS_V(16) RS_V(32) RS_V(64)
+                }
 #undef RS_V
                 return hipErrorInvalidValue;
             }
         }
-        if constexpr (AM != 2) {
+        if constexpr (AM <= 1) {
             constexpr bool AL = AM == 1;
 #define REC_V(PQ_, NB_)                                                                              \
     if (pq == PQ_ && nbuf == NB_) {                                                                  \
@@ -482,11 +497,13 @@ hipError_t launch_recover(int nl, int am, uint8_t *data, uint32_t L, const RecPr
                           const MulTab *tw, hipStream_t s, int pq, int nbuf, uint32_t cus) {
     if (pq == 0) pq = (am != 0 || nl < 8) ? default_pq(1, nl) : 32;  // byte path: 32 quads at 256 points
     if (nbuf == 0) nbuf = am != 0 ? default_nbuf(1, nl) : 2;
+    if (am == 3 && nl > 4) pq = 64, nbuf = 1;  // the only wide variant
     if (!variant_ok(nl, pq, nbuf)) return hipErrorInvalidValue;
 #define RC_CASE(K)                                                                                  \
     case K:                                                                                         \
         return am == 1 ? rec_one<K, 1>(data, L, prep, tw, s, pq, nbuf, cus)                         \
              : am == 2 ? rec_one<K, 2>(data, L, prep, tw, s, pq, nbuf, cus)                         \
+             : am == 3 ? rec_one<K, 3>(data, L, prep, tw, s, pq, nbuf, cus)                         \
                        : rec_one<K, 0>(data, L, prep, tw, s, pq, nbuf, cus);
     switch (nl) {
         RC_CASE(1) RC_CASE(2) RC_CASE(3) RC_CASE(4) RC_CASE(5) RC_CASE(6) RC_CASE(7) RC_CASE(8)
@@ -505,9 +522,10 @@ static hipError_t rec_fused_one(uint8_t *data, uint32_t L, const PosArg &pos, ui
         hipLaunchKernelGGL((k_recover_small_fused<NL, AM>), dim3(grid), dim3(256), 0, s, data, L, pos, e, gtab, tw);
     } else if constexpr (AM != 0) {
         constexpr int PQ = default_pq(1, NL);
+        static_assert(AM != 3 || PQ % 64 == 0, "wide rows need one shard group per wave");
         const uint32_t grid = ceil_div(quads, PQ);
         if (grid > max_groups) return hipErrorNotSupported;
-        if (oob_ok(NL, L))
+        if (oob_ok(NL, AM, L))
             hipLaunchKernelGGL((k_recover_stream_fused<NL, PQ, true, AM>), dim3(grid), dim3(Tile<NL, PQ>::THREADS), 0,
                                s, data, L, grid, pos, e, gtab, tw);
         else
@@ -529,6 +547,7 @@ hipError_t launch_recover_fused(int nl, int am, uint8_t *data, uint32_t L, const
     case K:                                                                                         \
         return am == 1 ? rec_fused_one<K, 1>(data, L, pos, e, gtab, tw, s, max_groups)              \
              : am == 2 ? rec_fused_one<K, 2>(data, L, pos, e, gtab, tw, s, max_groups)              \
+             : am == 3 ? rec_fused_one<K, 3>(data, L, pos, e, gtab, tw, s, max_groups)              \
                        : rec_fused_one<K, 0>(data, L, pos, e, gtab, tw, s, max_groups);
     switch (nl) {
         RF_CASE(1) RF_CASE(2) RF_CASE(3) RF_CASE(4) RF_CASE(5) RF_CASE(6) RF_CASE(7) RF_CASE(8)
@@ -537,24 +556,27 @@ hipError_t launch_recover_fused(int nl, int am, uint8_t *data, uint32_t L, const
 #undef RF_CASE
 }
 
-template <int TL, bool AL>
+template <int TL, int AM>
 static hipError_t rcp_one(uint8_t *data, uint32_t L, const MulTab *tw, hipStream_t s) {
     const uint64_t quads = ((uint64_t)L + 3) / 4;
     if constexpr (TL <= 4) {
-        hipLaunchKernelGGL((k_recompute_small<TL, AL>), dim3(ceil_div(quads, 256)), dim3(256), 0, s,
+        hipLaunchKernelGGL((k_recompute_small<TL, AM>), dim3(ceil_div(quads, 256)), dim3(256), 0, s,
                            data, L, tw);
     } else {
         constexpr int PQ = tile_pq<TL>();
-        hipLaunchKernelGGL((k_recompute_tile<TL, PQ, AL>), dim3(ceil_div(quads, PQ)),
+        static_assert(AM != 3 || PQ % 64 == 0, "wide rows need one shard group per wave");
+        hipLaunchKernelGGL((k_recompute_tile<TL, PQ, AM>), dim3(ceil_div(quads, PQ)),
                            dim3(Tile<TL, PQ>::THREADS), 0, s, data, L, tw);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_recompute(int tl, bool al, uint8_t *data, uint32_t L, const MulTab *tw,
+hipError_t launch_recompute(int tl, int am, uint8_t *data, uint32_t L, const MulTab *tw,
                             hipStream_t s) {
-#define RP_CASE(K) \
-    case K: return al ? rcp_one<K, true>(data, L, tw, s) : rcp_one<K, false>(data, L, tw, s);
+#define RP_CASE(K)                                                 \
+    case K: return am == 3 ? rcp_one<K, 3>(data, L, tw, s)         \
+                 : am == 1 ? rcp_one<K, 1>(data, L, tw, s)         \
+                           : rcp_one<K, 0>(data, L, tw, s);
     switch (tl) {
         RP_CASE(0) RP_CASE(1) RP_CASE(2) RP_CASE(3) RP_CASE(4) RP_CASE(5) RP_CASE(6) RP_CASE(7)
         default: return hipErrorInvalidValue;

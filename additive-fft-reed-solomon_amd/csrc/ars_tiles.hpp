@@ -622,6 +622,8 @@ __device__ __forceinline__ void recover_prep_body(RecPrep *__restrict__ out, uin
 //      last quad of a row hold bytes of the neighbouring rows or past the
 //      buffer; the transform computes them but the lane writes only the bytes
 //      of its own row (`bm`).
+//   3  wide dwords: mode 1's conditions for codewords of 4 GiB and more (the
+//      specialisation below).
 // A bool AL converts to 0 or 1.
 template <int AM, bool UNI = false>
 struct BufIO {
@@ -670,6 +672,42 @@ struct BufIO {
     }
 };
 
+// Access mode 3: rows whose byte offsets pass 2^32 (n * shard_len >= 4 GiB).
+// Every access goes through a resource based at its own row, (g + j) * L
+// bytes past the buffer (a 64-bit address), sized to the row, so the lane's
+// column is the only offset and any shard_len below 4 GiB works.  The kernels
+// use this mode only where g is wave-uniform (one shard group per wave, chunk
+// and block indices), so the row address and the resource are formed on the
+// scalar unit.  Every row the bodies address exists in these launches; a lane
+// past the row's end reads column 0 and stores nothing (as in mode 1).
+template <bool UNI>
+struct BufIO<3, UNI> {
+    static constexpr bool AL = true;
+    static constexpr bool kUniformGroups = UNI;
+    const uint8_t *in;
+    uint8_t *out;
+    uint32_t L, off, nb;
+    __device__ __forceinline__ Rows row(const void *base, uint32_t g, int j) const {
+        return make_rows(static_cast<const uint8_t *>(base) + (uint64_t)(g + (uint32_t)j) * L, L);
+    }
+    __device__ __forceinline__ uint32_t ld_in(uint32_t g, int j) const {
+        return __builtin_amdgcn_raw_buffer_load_b32(row(in, g, j).r, (int)off, 0, 0);
+    }
+    __device__ __forceinline__ uint32_t ld_out(uint32_t g, int j) const {
+        return __builtin_amdgcn_raw_buffer_load_b32(row(out, g, j).r, (int)off, 0, 0);
+    }
+    __device__ __forceinline__ void st_out(uint32_t g, int j, uint32_t v) const {
+        if (nb) __builtin_amdgcn_raw_buffer_store_b32(v, row(out, g, j).r, (int)off, 0, 0);
+    }
+    // input row g + j, or 0 if `zero` (read at offset L, the row resource's end)
+    __device__ __forceinline__ uint32_t ld_in_or0(uint32_t g, int j, bool zero) const {
+        return __builtin_amdgcn_raw_buffer_load_b32(row(in, g, j).r, (int)(zero ? L : off), 0, 0);
+    }
+    __device__ __forceinline__ void xor_out(uint32_t g, int j, uint32_t v) const {
+        if (nb) ars_buffer_atomic_xor((int)v, row(out, g, j).r, (int)off, 0, 0);
+    }
+};
+
 // Byte offset of p past the dword boundary below it (access mode 2's r).
 __device__ __forceinline__ uint32_t dword_shift(const void *p) { return (uint32_t)(reinterpret_cast<uintptr_t>(p) & 3u); }
 
@@ -679,7 +717,15 @@ __device__ __forceinline__ BufIO<AM, UNI> make_bufio(const void *in, uint32_t in
     BufIO<AM, UNI> io;
     io.L = L;
     const uint32_t col = quad * 4u;
-    if constexpr (AM == 2) {
+    if constexpr (AM == 3) {
+        // the byte counts are not used: every row is sized L (see BufIO<3>)
+        (void)in_bytes;
+        (void)out_bytes;
+        io.in = static_cast<const uint8_t *>(in);
+        io.out = static_cast<uint8_t *>(out);
+        io.nb = col < L ? 4u : 0u;
+        io.off = io.nb ? col : 0u;
+    } else if constexpr (AM == 2) {
         // rows addressed from the dword boundary below each base; both bases
         // have the same r (the launchers check it)
         const uint32_t r = dword_shift(in);

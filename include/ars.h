@@ -18,8 +18,12 @@
  * code (the unique systematic codeword with deg f < k) by a different route.
  *
  * All pointers named d_* are device pointers (HBM).  shard_len may be any
- * value >= 1 with n * shard_len < 4 GiB; shard_len % 4 == 0 with 4-byte
- * aligned pointers takes the dword path.  No entry point allocates on the hot path except
+ * value in [1, 4 GiB); shard_len % 4 == 0 with 4-byte aligned pointers takes
+ * the dword path.  Codewords of 4 GiB and more (n * shard_len >= 2^32) run on
+ * the dword path only (ars_encode, ars_recover, ars_recover_with_plan,
+ * ars_recover_noalloc, ars_recompute_data: ARS_E_UNSUPPORTED otherwise) and on
+ * any buffers through the *_host variants; ars_decode, batches and the server
+ * keep n * shard_len below 4 GiB (ARS_E_INVAL).  No entry point allocates on the hot path except
  * ars_recover (a 6.4 KB erasure plan buffer, allocated on a context's first
  * call on each stream and kept until ars_destroy; per call, stream-ordered,
  * on hipStreamPerThread, under graph capture and past 64 streams; launches of
@@ -50,7 +54,8 @@ extern "C" {
 #define ARS_E_NULL (-4)        /* null pointer where data is required */
 #define ARS_E_HIP (-5)         /* HIP runtime error (message: ars_last_error) */
 #define ARS_E_NOMEM (-6)       /* device or host allocation failed */
-#define ARS_E_UNSUPPORTED (-7) /* neither T nor k is a power of two */
+#define ARS_E_UNSUPPORTED (-7) /* neither T nor k is a power of two, or a codeword of
+                                  4 GiB or more off the dword path */
 #define ARS_E_BUDGET (-8)      /* serving: request exceeds the HBM budget */
 #define ARS_E_STATE (-9)       /* serving / batch object used out of order */
 

@@ -98,16 +98,29 @@ int main() {
 
 
 def test_codeword_size_limit_without_gpu():
-    """n * shard_len must stay below 4 GiB (32-bit buffer offsets): a larger
-    codeword is rejected with ARS_E_INVAL by the argument checks, before the
-    context is used or any HIP call is made."""
+    """Codewords of 4 GiB and more (n * shard_len >= 2^32) run only on 4-byte
+    aligned device buffers with shard_len % 4 == 0; shard_len itself stays
+    below 4 GiB; decode keeps n * shard_len below 4 GiB.  Each case is
+    rejected by the argument checks, before the context is used or any HIP
+    call is made."""
     ars = _pkg.load()
     if not os.path.exists(ars.LIB_PATH):
         pytest.skip("libars.so not built")
     lib = ars.lib()
-    bogus_ctx = 1  # never dereferenced: the size check fails first
+    bogus_ctx = 1  # never dereferenced: the checks fail first
     L = 1 << 24    # 256 * 16 MiB = 4 GiB
-    assert lib.ars_encode(bogus_ctx, 256, 128, 1, 1, L, 0, None) == ars.ARS_E_INVAL
-    assert lib.ars_recover(bogus_ctx, 256, 128, 1, L, None, 0, None) == ars.ARS_E_INVAL
-    assert lib.ars_recompute_data(bogus_ctx, 256, 128, 1, L, None) == ars.ARS_E_INVAL
+    # buffers 1 byte past alignment: no wide form
+    assert lib.ars_encode(bogus_ctx, 256, 128, 1, 1, L, 0, None) == ars.ARS_E_UNSUPPORTED
+    assert "4 GiB" in lib.ars_last_error().decode()
+    assert lib.ars_recover(bogus_ctx, 256, 128, 1, L, None, 0, None) == ars.ARS_E_UNSUPPORTED
+    assert lib.ars_recover_noalloc(bogus_ctx, 256, 128, 1, L, None, 0, None) == ars.ARS_E_UNSUPPORTED
+    assert lib.ars_recover_with_plan(bogus_ctx, 256, 128, 1, L, 1, None) == ars.ARS_E_UNSUPPORTED
+    assert lib.ars_recompute_data(bogus_ctx, 256, 128, 1, L, None) == ars.ARS_E_UNSUPPORTED
+    # shard_len not a multiple of 4
+    assert lib.ars_encode(bogus_ctx, 256, 128, 4, 8, L + 2, 0, None) == ars.ARS_E_UNSUPPORTED
+    # shard_len of 4 GiB
+    assert lib.ars_encode(bogus_ctx, 2, 1, 4, 8, 1 << 32, 0, None) == ars.ARS_E_INVAL
+    assert "shard_len" in lib.ars_last_error().decode()
+    # decode has no wide form
+    assert lib.ars_decode(bogus_ctx, 256, 128, 4, L, 0, None, None, None) == ars.ARS_E_INVAL
     assert "4 GiB" in lib.ars_last_error().decode()
