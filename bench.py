@@ -213,7 +213,7 @@ def valu_roofline(kernel, seconds, rates):
     name, rec = _profile_entry("valu_256x128_1MiB.json", kernel)
     if rec is None or rates is None:
         return None
-    fam = next((v["value"] for k, v in perm.items() if name.startswith(k)), None)
+    fam = perm[name]["value"] if name in perm else None
     if fam is None:
         return None
     achieved = rec["valu_instr_per_wave"] * rec["waves"] / seconds
@@ -225,6 +225,265 @@ def valu_roofline(kernel, seconds, rates):
             # straight-line code (its instruction mix is that butterfly's)
             "frac_of_straight_line_butterfly_rate": round(achieved / rates["butterfly_straight_line"], 4),
             "peak_from": "live v_xor_b32 and v_perm_b32 issue rates weighted by the kernel's v_perm share"}
+
+
+# ---------------------------------------------------------------------------
+# rocprofv3 summaries.  A profiled run writes CSV directories (one per
+# PROFILE_RUNS entry, `profile_command`); `--summarize-rocprof DIR` turns them
+# into the profiles/*.md tables and the profiles/*.json records that the
+# roofline fields above read, stamped with the device-source SHA the run was
+# built from and the git commit.
+
+PROFILE_HEAD = "--no-cpu --no-batched --no-served --no-host --no-decode --no-grid --no-c5 --no-wide"
+PROFILE_RUNS = {
+    "trace": ("--kernel-trace --stats", "--steps 5 --warmup 2 " + PROFILE_HEAD),
+    "fetch": ("--pmc FETCH_SIZE", "--steps 2 --warmup 1 " + PROFILE_HEAD),
+    "write": ("--pmc WRITE_SIZE", "--steps 2 --warmup 1 " + PROFILE_HEAD),
+    "valu": ("--pmc SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_WAVES SQ_BUSY_CYCLES GRBM_GUI_ACTIVE",
+             "--steps 2 --warmup 1 " + PROFILE_HEAD),
+    "lds": ("--pmc SQ_LDS_IDX_ACTIVE SQ_LDS_BANK_CONFLICT SQ_LDS_DATA_FIFO_FULL SQ_LDS_CMD_FIFO_FULL "
+            "SQ_INSTS_LDS SQ_WAVES SQ_WAIT_INST_LDS SQ_ACTIVE_INST_LDS GRBM_GUI_ACTIVE",
+            "--steps 2 --warmup 1 " + PROFILE_HEAD),
+    "batched": ("--kernel-trace --stats",
+                "--steps 2 --warmup 1 --no-cpu --no-served --no-host --no-decode --no-grid --no-c5 --no-wide"),
+    "decode": ("--kernel-trace --stats",
+               "--steps 2 --warmup 1 --no-cpu --no-batched --no-served --no-host --no-grid --no-c5 --no-wide"),
+}
+
+
+def profile_command(name, outdir, repo="$R"):
+    prof, args = PROFILE_RUNS[name]
+    return f"rocprofv3 {prof} -f csv -d {outdir}/{name} -o run -- python3 {repo}/bench.py {args}"
+
+
+def _kname(raw):
+    """rocprofv3's kernel name without return type, argument list and the
+    library's namespace."""
+    s = raw.strip()
+    if s.startswith("void "):
+        s = s[5:]
+    depth = 0
+    for i, ch in enumerate(s):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            s = s[:i]
+            break
+    return s.replace("ars::", "").strip()
+
+
+def _rocprof_csv(d, suffix):
+    import csv
+    import glob
+    out = []
+    for f in sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True)):
+        with open(f, newline="") as fh:
+            out += [dict(r, _file=f) for r in csv.DictReader(fh)]
+    return out
+
+
+def _dispatches(d):
+    """{(file, dispatch id): {"kernel", "grid", counter: value}} of a --pmc run."""
+    per = {}
+    for r in _rocprof_csv(d, "counter_collection.csv"):
+        key = (r["_file"], int(r["Dispatch_Id"]))
+        e = per.setdefault(key, {"kernel": _kname(r["Kernel_Name"]), "grid": int(float(r.get("Grid_Size") or 0))})
+        e[r["Counter_Name"]] = e.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return per
+
+
+def _by_kernel(per, counters):
+    """{kernel: {"dispatches": c, counter: mean per dispatch}}."""
+    acc = {}
+    for e in per.values():
+        if not all(c in e for c in counters):
+            continue
+        a = acc.setdefault(e["kernel"], {"dispatches": 0, **{c: 0.0 for c in counters}})
+        a["dispatches"] += 1
+        for c in counters:
+            a[c] += e[c]
+    for a in acc.values():
+        for c in counters:
+            a[c] /= a["dispatches"]
+    return acc
+
+
+def _mangle(kernel):
+    """Itanium name of a namespace-ars kernel template with int / bool
+    arguments, e.g. k_recover_stream<8, 64, true, 1>."""
+    base, _, args = kernel.partition("<")
+    enc = ""
+    for a in args.rstrip(">").split(","):
+        a = a.strip()
+        enc += "Lb1E" if a == "true" else "Lb0E" if a == "false" else f"Li{int(a)}E"
+    return f"_ZN3ars{len(base)}{base}I{enc}EE"
+
+
+def _perm_fractions(kernels):
+    """v_perm_b32 share of the VALU instructions in each kernel's gfx950
+    assembly (static count over the kernel body, built here with hipcc)."""
+    import re
+    import subprocess
+    import tempfile
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    src = os.path.join(ROOT, "additive-fft-reed-solomon_amd", "csrc", "ars_kernels.hip")
+    with tempfile.TemporaryDirectory() as td:
+        dst = os.path.join(td, "k.s")
+        subprocess.run([hipcc, "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Xclang", "-target-feature",
+                        "-Xclang", "-packed-fp32-ops", "-S", "--cuda-device-only", src, "-o", dst],
+                       check=True, capture_output=True)
+        asm = open(dst).read().split("\n")
+    starts = [i for i, l in enumerate(asm) if re.match(r"^_Z[A-Za-z0-9_]+:", l)] + [len(asm)]
+    bodies = {asm[a].split(":")[0]: asm[a:b] for a, b in zip(starts, starts[1:])}
+    out = {}
+    for k in kernels:
+        body = bodies.get(_mangle(k))
+        if body is None:
+            continue
+        valu = sum(1 for l in body if re.match(r"^\s+v_", l))
+        perm = sum(1 for l in body if re.match(r"^\s+v_perm_b32", l))
+        if valu:
+            out[k] = {"value": round(perm / valu, 4),
+                      "source": f"{perm} v_perm_b32 of {valu} VALU instructions in the gfx950 assembly of {k}"}
+    return out
+
+
+def _headline_kernel(k):
+    return k.startswith(("k_encode_tile<7,", "k_recover_tile<8,", "k_recover_stream<8,"))
+
+
+def summarize_rocprof(d):
+    """Writes the profiles/ records of the runs under d (see PROFILE_RUNS)."""
+    import subprocess
+    sha = device_source_sha()
+    with open(os.path.join(d, "sha.txt")) as f:
+        run_sha = f.read().strip()
+    if run_sha != sha:
+        raise SystemExit(f"profiles in {d} were made from device sources {run_sha[:12]}, this tree has {sha[:12]}")
+    commit = subprocess.run(["git", "-C", ROOT, "rev-parse", "HEAD"], capture_output=True, text=True).stdout.strip()
+    prof = os.path.join(ROOT, "profiles")
+    n, k, L, e = 256, 128, 1 << 20, 128
+    alg = {"encode": n * L, "recover": (n + e) * L}
+    stamp = {"device_source_sha": sha, "commit": commit}
+    wl = "(n=256,k=128), L=1 MiB, e=128"
+    how = ("Run on one MI355X from /tmp with TMPDIR=/tmp, repo at $R; each counter set in a run of its own, "
+           "every pass under `timeout -s KILL 60`.  Summarised by `python bench.py --summarize-rocprof DIR`, "
+           f"device sources {sha[:16]} (bench.device_source_sha), commit {commit[:12]}.")
+
+    def md(fname, title, cmds, body):
+        with open(os.path.join(prof, fname), "w") as f:
+            f.write(f"# {title}\n\n{how}\n\n```\n" + "\n".join(cmds) + "\n```\n\n" + body.rstrip() + "\n")
+
+    # kernel stats of the headline run, the batched run and the decode run
+    def stats(run, fname, title, note, keep):
+        rows = _rocprof_csv(os.path.join(d, run), "kernel_stats.csv")
+        recs = {}
+        for r in rows:
+            kn = _kname(r["Name"])
+            recs[kn] = {"calls": int(r["Calls"]), "avg_us": round(float(r["AverageNs"]) / 1e3, 2),
+                        "min_us": round(float(r["MinNs"]) / 1e3, 2), "max_us": round(float(r["MaxNs"]) / 1e3, 2),
+                        "total_ms": round(float(r["TotalDurationNs"]) / 1e6, 3)}
+        order = sorted(recs, key=lambda x: -recs[x]["total_ms"])
+        t = "| Kernel | Calls | Avg (us) | Min (us) | Max (us) | Total (ms) |\n|---|---|---|---|---|---|\n"
+        t += "".join(f"| `{x}` | {recs[x]['calls']} | {recs[x]['avg_us']} | {recs[x]['min_us']} | "
+                     f"{recs[x]['max_us']} | {recs[x]['total_ms']} |\n" for x in order[:keep])
+        md(fname + ".md", title, [profile_command(run, "/tmp/prof")], note + "\n\n" + t)
+        with open(os.path.join(prof, fname + ".json"), "w") as f:
+            json.dump({"workload": title, **stamp, "kernels": {x: recs[x] for x in order}}, f, indent=1)
+        return recs
+
+    stats("trace", "kernel_stats_256x128_1MiB", f"Kernel stats: {wl}, encode + recover on one MI355X",
+          "The run launches the VALU issue-rate probes (`k_xor`, `k_perm`, `k_butterfly`, `k_butterfly_long`), "
+          "the autotuner (every tile variant, 6 launches each), the correctness gate, warm-up, 5 timed steps, "
+          "and max(steps, 10) + 1 launches each of encode alone and recover-with-plan alone. torch kernels are "
+          "the bench's data set-up, correctness gate and copy reference.", 30)
+    if os.path.isdir(os.path.join(d, "batched")):
+        stats("batched", "kernel_stats_batched_mixed_4096", "Kernel stats: batched mixed-size workload vs one "
+              "launch per codeword", "Workload: 4096 codewords, n ~ U{32,64,128,256}, k = n/2, shard_len "
+              "log-uniform 64 B - 64 KiB, encode then recover of e ~ U{0..T} erasures, as one batch and as one "
+              "call per codeword; the run also holds the (256,128) headline steps and the autotuner.", 30)
+    if os.path.isdir(os.path.join(d, "decode")):
+        stats("decode", "kernel_stats_decode_256x128", "Kernel stats: sharded error decoding, (n=256,k=128)",
+              "The run holds the bench's decode line (clean scrub, corrupted columns, formal-derivative mode, "
+              "key equations on the GPU) beside the headline steps.", 30)
+
+    # HBM traffic: FETCH_SIZE and WRITE_SIZE passes, calibrated on the
+    # largest device-to-device copy (n*L bytes read and written)
+    fetch, write = _dispatches(os.path.join(d, "fetch")), _dispatches(os.path.join(d, "write"))
+    cal = {}
+    for name, per, ctr in (("fetch", fetch, "FETCH_SIZE"), ("write", write, "WRITE_SIZE")):
+        cps = [x for x in per.values() if "copyBuffer" in x["kernel"] and ctr in x]
+        g = max(x["grid"] for x in cps)
+        vals = sorted(x[ctr] for x in cps if x["grid"] == g)
+        cal[name] = {"kernel": cps[0]["kernel"], "grid": g, "dispatches": len(vals),
+                     f"{ctr}_KiB": round(vals[len(vals) // 2], 1), "true_KiB": n * L / 1024,
+                     "factor": round(n * L / 1024 / vals[len(vals) // 2], 4)}
+    if abs(cal["write"]["factor"] - 1.0) > 0.02:
+        raise SystemExit(f"the largest copy is not n*L bytes: {cal['write']}")
+    fk, wk = _by_kernel(fetch, ["FETCH_SIZE"]), _by_kernel(write, ["WRITE_SIZE"])
+    tr = {}
+    for kn in sorted(set(fk) & set(wk)):
+        if not _headline_kernel(kn):
+            continue
+        a = alg["encode" if kn.startswith("k_encode") else "recover"]
+        t = (fk[kn]["FETCH_SIZE"] * cal["fetch"]["factor"] + wk[kn]["WRITE_SIZE"] * cal["write"]["factor"]) * 1024
+        tr[kn] = {"FETCH_SIZE_KiB": round(fk[kn]["FETCH_SIZE"], 1), "WRITE_SIZE_KiB": round(wk[kn]["WRITE_SIZE"], 1),
+                  "dispatches": fk[kn]["dispatches"], "traffic_bytes": int(t), "algorithmic_bytes": a,
+                  "traffic_over_algorithmic": round(t / a, 4)}
+    with open(os.path.join(prof, "traffic_256x128_1MiB.json"), "w") as f:
+        json.dump({"workload": wl, **stamp, "fetch_write_units": "KiB per dispatch", "calibration": cal,
+                   "kernels": tr}, f, indent=1)
+    t = ("| Kernel | Dispatches | FETCH_SIZE (KiB) | WRITE_SIZE (KiB) | traffic (bytes) | algorithmic (bytes) | ratio |\n"
+         "|---|---|---|---|---|---|---|\n")
+    t += "".join(f"| `{x}` | {v['dispatches']} | {v['FETCH_SIZE_KiB']} | {v['WRITE_SIZE_KiB']} | {v['traffic_bytes']} "
+                 f"| {v['algorithmic_bytes']} | {v['traffic_over_algorithmic']} |\n" for x, v in tr.items())
+    cf, cw = cal["fetch"], cal["write"]
+    md("hbm_traffic_256x128_1MiB.md", f"HBM traffic per launch: {wl}",
+       [profile_command("fetch", "/tmp/prof"), profile_command("write", "/tmp/prof")],
+       f"**Calibration** on the largest device-to-device copy of the run (`{cf['kernel']}`, {n * L} bytes read "
+       f"and written, median of {cf['dispatches']} dispatches): FETCH_SIZE {cf['FETCH_SIZE_KiB']} KiB, factor "
+       f"{cf['factor']}; WRITE_SIZE {cw['WRITE_SIZE_KiB']} KiB, factor {cw['factor']}.  traffic = (FETCH_SIZE x "
+       f"{cf['factor']} + WRITE_SIZE x {cw['factor']}) KiB per dispatch.  Algorithmic bytes: n*L for encode "
+       "(k*L read, T*L written), (n+e)*L for recover (n*L read, e*L written).  Every tile variant the autotuner "
+       "launches is listed.\n\n" + t)
+
+    # VALU issue and LDS activity
+    vk = _by_kernel(_dispatches(os.path.join(d, "valu")),
+                    ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_WAVES", "SQ_BUSY_CYCLES", "GRBM_GUI_ACTIVE"])
+    heads = [x for x in sorted(vk) if _headline_kernel(x)]
+    perm = _perm_fractions(heads)
+    vr = {x: {"waves": int(vk[x]["SQ_WAVES"]), "valu_instr_per_wave": round(vk[x]["SQ_INSTS_VALU"] / vk[x]["SQ_WAVES"], 1),
+              "busy_cycles_per_xcd": int(vk[x]["GRBM_GUI_ACTIVE"] / 8), "dispatches": vk[x]["dispatches"]}
+          for x in heads}
+    with open(os.path.join(prof, "valu_256x128_1MiB.json"), "w") as f:
+        json.dump({"workload": wl, **stamp, "perm_fraction": perm, "kernels": vr}, f, indent=1)
+    t = "| Kernel | Dispatches | Waves | VALU instr / wave | busy cycles / XCD | v_perm share (static) |\n|---|---|---|---|---|---|\n"
+    t += "".join(f"| `{x}` | {v['dispatches']} | {v['waves']} | {v['valu_instr_per_wave']} | {v['busy_cycles_per_xcd']} "
+                 f"| {perm.get(x, {}).get('value', '')} |\n" for x, v in vr.items())
+    md("valu_256x128_1MiB.md", f"VALU issue: {wl}", [profile_command("valu", "/tmp/prof")],
+       "SQ_INSTS_VALU counts wave64 VALU instructions over the device; GRBM_GUI_ACTIVE is summed over the 8 "
+       "XCDs (busy cycles per XCD = value / 8).  The v_perm share is counted in the kernel's gfx950 assembly "
+       "(each GF(2^8) multiply of four bytes is three v_perm_b32 table lookups); bench.py weights the live "
+       "v_xor_b32 and v_perm_b32 issue rates by it for the VALU roofline.\n\n" + t)
+    lc = ["SQ_LDS_IDX_ACTIVE", "SQ_LDS_BANK_CONFLICT", "SQ_LDS_DATA_FIFO_FULL", "SQ_LDS_CMD_FIFO_FULL", "SQ_INSTS_LDS",
+          "SQ_WAVES", "SQ_WAIT_INST_LDS", "SQ_ACTIVE_INST_LDS", "GRBM_GUI_ACTIVE"]
+    lk = _by_kernel(_dispatches(os.path.join(d, "lds")), lc)
+    t = ("| Kernel | Dispatches | Waves | Cycles / XCD | LDS busy / CU | LDS busy | Bank-conflict cycles / CU | "
+         "LDS instr / wave |\n|---|---|---|---|---|---|---|---|\n")
+    for x in sorted(lk):
+        if not _headline_kernel(x):
+            continue
+        v = lk[x]
+        cyc = v["GRBM_GUI_ACTIVE"] / 8
+        busy = v["SQ_LDS_IDX_ACTIVE"] / 256
+        t += (f"| `{x}` | {v['dispatches']} | {int(v['SQ_WAVES'])} | {int(cyc)} | {int(busy)} | {busy / cyc:.3f} | "
+              f"{int(v['SQ_LDS_BANK_CONFLICT'] / 256)} | {v['SQ_INSTS_LDS'] / v['SQ_WAVES']:.1f} |\n")
+    md("lds_256x128_1MiB.md", f"LDS activity per launch: {wl}", [profile_command("lds", "/tmp/prof")],
+       "Per-CU figures are chip sums / 256 CUs; LDS busy = SQ_LDS_IDX_ACTIVE per CU over the cycles per XCD "
+       "(rocprofiler-sdk's LdsUtil expression for gfx950).\n\n" + t)
+    print(f"wrote profiles from {d} (device sources {sha[:12]}, commit {commit[:12]})")
 
 
 def c3_workload(np, count=4096, seed=44):
@@ -687,6 +946,42 @@ def bench_c5(ars, torch, np, stream, seed=46):
             "shapes": shapes, "tiny_32x16_64B": tiny}
 
 
+def bench_wide(ars, torch, np, stream, seed=47):
+    """(256,128) with 24 MiB shards, a 6 GiB codeword (n * shard_len >= 2^32:
+    each row addressed through its own resource), beside 16 MiB - 64 shards
+    just under 4 GiB (32-bit offsets): encode and recover (e = 128) GB/s.
+    Each recover is checked to return its codeword before it is timed."""
+    ctx = ars.default_context(0)
+    rng = np.random.default_rng(seed)
+    n, k = 256, 128
+    T = n - k
+    res = {}
+    for name, L in (("narrow_4GiB_minus", (16 << 20) - 64), ("wide_6GiB", 24 << 20)):
+        cw = torch.randint(0, 256, (n * L,), dtype=torch.uint8, device="cuda")
+        par, msg = cw[:T * L], cw[T * L:]
+        ctx.encode(n, k, msg.data_ptr(), par.data_ptr(), L, 0, stream)
+        torch.cuda.synchronize()
+        good = cw.clone()
+        pos = np.sort(rng.permutation(n)[:T]).astype(np.uint8)
+        cw.view(n, L)[torch.from_numpy(pos.astype(np.int64)).cuda()] = 0
+        torch.cuda.synchronize()
+        ctx.recover(n, k, cw.data_ptr(), L, pos, stream)
+        torch.cuda.synchronize()
+        if not torch.equal(cw, good):
+            raise RuntimeError(f"{name}: recover differs")
+        del good
+        enc_s, _ = _per_call(torch, stream, lambda: ctx.encode(n, k, msg.data_ptr(), par.data_ptr(), L, 0, stream),
+                             budget_s=0.1, sync_reps=3)
+        rec_s, _ = _per_call(torch, stream, lambda: ctx.recover(n, k, cw.data_ptr(), L, pos, stream),
+                             budget_s=0.1, sync_reps=3)
+        res[name] = {"shard_len": L, "codeword_bytes": n * L,
+                     "encode": {"ms": round(enc_s / 1e3, 3), "GBps": round(n * L / enc_s / 1e3, 1)},
+                     "recover": {"ms": round(rec_s / 1e3, 3), "GBps": round((n + T) * L / rec_s / 1e3, 1)}}
+        del cw, par, msg
+        torch.cuda.empty_cache()
+    return {"workload": "(256,128), e = 128: 16 MiB - 64 B shards (n*L < 4 GiB) vs 24 MiB shards (6 GiB)", **res}
+
+
 def bench_host(ars, torch, np, n, k, L, e, reps=5, seed=46):
     """The host-slice entry points (the Rust binding's path): the same
     (n, k, L) codeword in pageable host memory, streamed through the GPU in
@@ -817,7 +1112,23 @@ def main():
     ap.add_argument("--no-decode", action="store_true")
     ap.add_argument("--no-grid", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--no-wide", action="store_true")
+    ap.add_argument("--summarize-rocprof", metavar="DIR",
+                    help="write profiles/ from the rocprofv3 CSV directories under DIR and exit")
+    ap.add_argument("--print-source-sha", action="store_true", help="print the device-source SHA and exit")
+    ap.add_argument("--print-profile-commands", metavar="DIR",
+                    help="print the rocprofv3 command of each profile pass, writing under DIR, and exit")
     args = ap.parse_args()
+    if args.print_source_sha:
+        print(device_source_sha())
+        return
+    if args.print_profile_commands:
+        for name in PROFILE_RUNS:
+            print(profile_command(name, args.print_profile_commands, ROOT))
+        return
+    if args.summarize_rocprof:
+        summarize_rocprof(args.summarize_rocprof)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every figure is "
               "per-GPU and the path shards by byte-column range with no exchange.", file=sys.stderr)
@@ -911,8 +1222,9 @@ def main():
     enc_gbps = n * L / s_enc / 1e9
     rec_gbps = (n + e) * L / s_rec / 1e9
     # the tile variants the autotuner kept (quads per workgroup, LDS buffers)
-    enc_name = f"k_encode_tile<7, {tune_enc['quads']}, true, false, {tune_enc['buffers']}, true>"
-    rec_name = (f"k_recover_stream<8, {tune_rec['quads']}, true>" if tune_rec["buffers"] == 1
+    # (names as rocprofv3 prints them: aligned 4-byte buffers, access mode 1)
+    enc_name = f"k_encode_tile<7, {tune_enc['quads']}, 1, false, {tune_enc['buffers']}, true>"
+    rec_name = (f"k_recover_stream<8, {tune_rec['quads']}, true, 1>" if tune_rec["buffers"] == 1
                 else f"k_recover_tile<8, {tune_rec['quads']}, true, {tune_rec['buffers']}>")
     if s_rec >= s_enc:
         dom, dom_bytes, dom_s = rec_name, (n + e) * L, s_rec
@@ -920,6 +1232,7 @@ def main():
         dom, dom_bytes, dom_s = enc_name, n * L, s_enc
     achieved = dom_bytes / dom_s / 1e9
     traffic = measured_traffic(dom)
+    _, stat = _profile_entry("kernel_stats_256x128_1MiB.json", dom)
     valu_rates = probe_valu_rates()
 
     out = {
@@ -940,7 +1253,8 @@ def main():
                    "n": n, "k": k, "shard_len": L, "erasures": e, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": traffic, "algorithmic_bytes": dom_bytes},
+                     "traffic": traffic, "algorithmic_bytes": dom_bytes, "launch_us": round(dom_s * 1e6, 2),
+                     "rocprof_avg_us": stat["avg_us"] if stat else None},
         "valu_roofline": valu_roofline(dom, dom_s, valu_rates),
         "valu_issue_rates": {k: round(v / 1e12, 3) for k, v in valu_rates.items()} if valu_rates else None,
         "encode": {"GBps": round(enc_gbps, 1), "us": round(s_enc * 1e6, 2),
@@ -972,6 +1286,9 @@ def main():
     if not args.no_c5:
         out["c5"] = bench_c5(ars, torch, np, stream)
         print(f"c5: {out['c5']}", file=sys.stderr, flush=True)
+    if not args.no_wide:
+        out["wide"] = bench_wide(ars, torch, np, stream)
+        print(f"wide: {out['wide']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, k, L, e, args.cpu_seconds)
     print(json.dumps(out), flush=True)
