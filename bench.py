@@ -244,6 +244,13 @@ PROFILE_RUNS = {
     "lds": ("--pmc SQ_LDS_IDX_ACTIVE SQ_LDS_BANK_CONFLICT SQ_LDS_DATA_FIFO_FULL SQ_LDS_CMD_FIFO_FULL "
             "SQ_INSTS_LDS SQ_WAVES SQ_WAIT_INST_LDS SQ_ACTIVE_INST_LDS GRBM_GUI_ACTIVE",
             "--steps 2 --warmup 1 " + PROFILE_HEAD),
+    # read requests by size (FETCH_SIZE's 128-byte term reads TCC_BUBBLE, which
+    # gfx950 does not count: hence its calibration), atomics, and DRAM requests
+    "rdsize": ("--pmc TCC_EA0_RDREQ TCC_EA0_RDREQ_32B TCC_EA0_RDREQ_64B TCC_EA0_RDREQ_128B",
+               "--steps 2 --warmup 1 " + PROFILE_HEAD),
+    "atomic": ("--pmc TCC_ATOMIC TCC_EA0_ATOMIC TCC_EA0_WRREQ TCC_EA0_WRREQ_64B", "--steps 2 --warmup 1 " + PROFILE_HEAD),
+    "dram": ("--pmc TCC_EA0_RDREQ_DRAM TCC_EA0_WRREQ_DRAM TCC_EA0_WRREQ_ATOMIC_DRAM TCC_EA0_RDREQ_DRAM_32B",
+             "--steps 2 --warmup 1 " + PROFILE_HEAD),
     "batched": ("--kernel-trace --stats",
                 "--steps 2 --warmup 1 --no-cpu --no-served --no-host --no-decode --no-grid --no-c5 --no-wide"),
     "decode": ("--kernel-trace --stats",
@@ -318,7 +325,7 @@ def _mangle(kernel):
     for a in args.rstrip(">").split(","):
         a = a.strip()
         enc += "Lb1E" if a == "true" else "Lb0E" if a == "false" else f"Li{int(a)}E"
-    return f"_ZN3ars{len(base)}{base}I{enc}EE"
+    return f"_ZN3ars{len(base)}{base}I{enc}E"
 
 
 def _perm_fractions(kernels):
@@ -339,7 +346,8 @@ def _perm_fractions(kernels):
     bodies = {asm[a].split(":")[0]: asm[a:b] for a, b in zip(starts, starts[1:])}
     out = {}
     for k in kernels:
-        body = bodies.get(_mangle(k))
+        # the symbol continues with the parameter types after the template arguments
+        body = next((b for sym, b in bodies.items() if sym.startswith(_mangle(k))), None)
         if body is None:
             continue
         valu = sum(1 for l in body if re.match(r"^\s+v_", l))
@@ -410,7 +418,10 @@ def summarize_rocprof(d):
               "key equations on the GPU) beside the headline steps.", 30)
 
     # HBM traffic: FETCH_SIZE and WRITE_SIZE passes, calibrated on the
-    # largest device-to-device copy (n*L bytes read and written)
+    # largest device-to-device copy (n*L bytes read and written); read
+    # requests counted by size (no calibration) as the cross-check; atomic
+    # requests, which the L2 forwards to the memory side, where each reads
+    # the line it modifies
     fetch, write = _dispatches(os.path.join(d, "fetch")), _dispatches(os.path.join(d, "write"))
     cal = {}
     for name, per, ctr in (("fetch", fetch, "FETCH_SIZE"), ("write", write, "WRITE_SIZE")):
@@ -423,31 +434,56 @@ def summarize_rocprof(d):
     if abs(cal["write"]["factor"] - 1.0) > 0.02:
         raise SystemExit(f"the largest copy is not n*L bytes: {cal['write']}")
     fk, wk = _by_kernel(fetch, ["FETCH_SIZE"]), _by_kernel(write, ["WRITE_SIZE"])
+    rq = _by_kernel(_dispatches(os.path.join(d, "rdsize")),
+                    ["TCC_EA0_RDREQ_32B", "TCC_EA0_RDREQ_64B", "TCC_EA0_RDREQ_128B"])
+    at = _by_kernel(_dispatches(os.path.join(d, "atomic")), ["TCC_ATOMIC", "TCC_EA0_ATOMIC"])
+    dr = _by_kernel(_dispatches(os.path.join(d, "dram")), ["TCC_EA0_WRREQ_ATOMIC_DRAM"])
     tr = {}
-    for kn in sorted(set(fk) & set(wk)):
+    for kn in sorted(set(fk) & set(wk) & set(rq) & set(at)):
         if not _headline_kernel(kn):
             continue
         a = alg["encode" if kn.startswith("k_encode") else "recover"]
-        t = (fk[kn]["FETCH_SIZE"] * cal["fetch"]["factor"] + wk[kn]["WRITE_SIZE"] * cal["write"]["factor"]) * 1024
+        rd = fk[kn]["FETCH_SIZE"] * cal["fetch"]["factor"] * 1024
+        wr = wk[kn]["WRITE_SIZE"] * cal["write"]["factor"] * 1024
+        rmw = 64 * at[kn]["TCC_EA0_ATOMIC"]
+        by_size = (32 * rq[kn]["TCC_EA0_RDREQ_32B"] + 64 * rq[kn]["TCC_EA0_RDREQ_64B"]
+                   + 128 * rq[kn]["TCC_EA0_RDREQ_128B"])
+        t = rd + wr + rmw
         tr[kn] = {"FETCH_SIZE_KiB": round(fk[kn]["FETCH_SIZE"], 1), "WRITE_SIZE_KiB": round(wk[kn]["WRITE_SIZE"], 1),
-                  "dispatches": fk[kn]["dispatches"], "traffic_bytes": int(t), "algorithmic_bytes": a,
+                  "read_bytes_by_request_size": int(by_size), "atomic_requests": int(at[kn]["TCC_EA0_ATOMIC"]),
+                  "atomic_requests_at_dram": int(dr[kn]["TCC_EA0_WRREQ_ATOMIC_DRAM"]) if kn in dr else None,
+                  "dispatches": fk[kn]["dispatches"], "read_bytes": int(rd), "write_bytes": int(wr),
+                  "atomic_rmw_read_bytes": int(rmw), "traffic_bytes": int(t), "algorithmic_bytes": a,
                   "traffic_over_algorithmic": round(t / a, 4)}
     with open(os.path.join(prof, "traffic_256x128_1MiB.json"), "w") as f:
         json.dump({"workload": wl, **stamp, "fetch_write_units": "KiB per dispatch", "calibration": cal,
-                   "kernels": tr}, f, indent=1)
-    t = ("| Kernel | Dispatches | FETCH_SIZE (KiB) | WRITE_SIZE (KiB) | traffic (bytes) | algorithmic (bytes) | ratio |\n"
-         "|---|---|---|---|---|---|---|\n")
-    t += "".join(f"| `{x}` | {v['dispatches']} | {v['FETCH_SIZE_KiB']} | {v['WRITE_SIZE_KiB']} | {v['traffic_bytes']} "
-                 f"| {v['algorithmic_bytes']} | {v['traffic_over_algorithmic']} |\n" for x, v in tr.items())
+                   "traffic": "FETCH_SIZE x fetch factor + WRITE_SIZE x write factor + 64 B per atomic request "
+                              "(read by the memory side that executes it)", "kernels": tr}, f, indent=1)
+    t = ("| Kernel | Disp. | FETCH_SIZE (KiB) | read bytes (FETCH x f) | read bytes by request size | WRITE_SIZE (KiB) | "
+         "atomic requests (at DRAM) | traffic (bytes) | algorithmic (bytes) | ratio |\n"
+         "|---|---|---|---|---|---|---|---|---|---|\n")
+    t += "".join(f"| `{x}` | {v['dispatches']} | {v['FETCH_SIZE_KiB']} | {v['read_bytes']} | "
+                 f"{v['read_bytes_by_request_size']} | {v['WRITE_SIZE_KiB']} | {v['atomic_requests']} "
+                 f"({v['atomic_requests_at_dram']}) | {v['traffic_bytes']} | {v['algorithmic_bytes']} | "
+                 f"{v['traffic_over_algorithmic']} |\n" for x, v in tr.items())
     cf, cw = cal["fetch"], cal["write"]
     md("hbm_traffic_256x128_1MiB.md", f"HBM traffic per launch: {wl}",
-       [profile_command("fetch", "/tmp/prof"), profile_command("write", "/tmp/prof")],
+       [profile_command(x, "/tmp/prof") for x in ("fetch", "write", "rdsize", "atomic", "dram")],
        f"**Calibration** on the largest device-to-device copy of the run (`{cf['kernel']}`, {n * L} bytes read "
        f"and written, median of {cf['dispatches']} dispatches): FETCH_SIZE {cf['FETCH_SIZE_KiB']} KiB, factor "
-       f"{cf['factor']}; WRITE_SIZE {cw['WRITE_SIZE_KiB']} KiB, factor {cw['factor']}.  traffic = (FETCH_SIZE x "
-       f"{cf['factor']} + WRITE_SIZE x {cw['factor']}) KiB per dispatch.  Algorithmic bytes: n*L for encode "
-       "(k*L read, T*L written), (n+e)*L for recover (n*L read, e*L written).  Every tile variant the autotuner "
-       "launches is listed.\n\n" + t)
+       f"{cf['factor']}; WRITE_SIZE {cw['WRITE_SIZE_KiB']} KiB, factor {cw['factor']}.  FETCH_SIZE's 128-byte "
+       "term counts TCC_BUBBLE, which stays 0 on gfx950 while its reads are 128-byte requests "
+       "(TCC_EA0_RDREQ_128B), so FETCH_SIZE shows half of them; the read bytes counted by request size "
+       "(32 x RDREQ_32B + 64 x RDREQ_64B + 128 x RDREQ_128B) need no factor and agree with FETCH_SIZE x factor.\n\n"
+       "**Atomics.** The streaming recover adds each recovered quad into its erased shard with "
+       "`buffer_atomic_xor`.  TCC_EA0_ATOMIC shows the L2 forwarding every one of them to the memory side as "
+       "a 64-byte request (TCC_EA0_WRREQ_ATOMIC_DRAM: all to DRAM), where the line is read, modified and "
+       "written; WRITE_SIZE counts the request, no TCC counter sees the read, so it is added as 64 bytes per "
+       "request.  The streaming kernel reads only the n - e shards it keeps (the erased ones are neither "
+       "loaded nor multiplied).\n\n"
+       "traffic = FETCH_SIZE x fetch factor + WRITE_SIZE x write factor + 64 x atomic requests.  Algorithmic "
+       "bytes: n*L for encode (k*L read, T*L written), (n+e)*L for recover (n*L read, e*L written).  Every tile "
+       "variant the autotuner launches is listed.\n\n" + t)
 
     # VALU issue and LDS activity
     vk = _by_kernel(_dispatches(os.path.join(d, "valu")),
