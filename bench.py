@@ -266,7 +266,7 @@ def profile_command(name, outdir, repo="$R"):
 def _kname(raw):
     """rocprofv3's kernel name without return type, argument list and the
     library's namespace."""
-    s = raw.strip()
+    s = raw.strip().replace("(anonymous namespace)", "{anon}")
     if s.startswith("void "):
         s = s[5:]
     depth = 0
