@@ -385,6 +385,17 @@ def summarize_rocprof(d):
             f.write(f"# {title}\n\n{how}\n\n```\n" + "\n".join(cmds) + "\n```\n\n" + body.rstrip() + "\n")
 
     # kernel stats of the headline run, the batched run and the decode run
+    def live(run):
+        """The bench line the profiled run printed: its own event timing of the
+        dominant kernel, encode and recover on the same box, for comparison."""
+        try:
+            line = [x for x in open(os.path.join(d, run + ".log")) if x.startswith("{")][-1]
+            b = json.loads(line)
+        except (OSError, IndexError, ValueError):
+            return None
+        return {"kernel": b["roofline"]["kernel"], "launch_us": b["roofline"].get("launch_us"),
+                "encode_us": b["encode"]["us"], "recover_us": b["recover"]["us"]}
+
     def stats(run, fname, title, note, keep):
         rows = _rocprof_csv(os.path.join(d, run), "kernel_stats.csv")
         recs = {}
@@ -397,9 +408,16 @@ def summarize_rocprof(d):
         t = "| Kernel | Calls | Avg (us) | Min (us) | Max (us) | Total (ms) |\n|---|---|---|---|---|---|\n"
         t += "".join(f"| `{x}` | {recs[x]['calls']} | {recs[x]['avg_us']} | {recs[x]['min_us']} | "
                      f"{recs[x]['max_us']} | {recs[x]['total_ms']} |\n" for x in order[:keep])
+        lv = live(run)
+        if lv:
+            note += (f"\n\nThe same run's bench line timed `{lv['kernel']}` at {lv['launch_us']} us per launch "
+                     f"(HIP events around back-to-back launches on its stream; encode {lv['encode_us']} us, recover "
+                     f"{lv['recover_us']} us).  The rocprof average below also holds the autotuner's and the "
+                     "correctness gate's launches.")
         md(fname + ".md", title, [profile_command(run, "/tmp/prof")], note + "\n\n" + t)
         with open(os.path.join(prof, fname + ".json"), "w") as f:
-            json.dump({"workload": title, **stamp, "kernels": {x: recs[x] for x in order}}, f, indent=1)
+            json.dump({"workload": title, **stamp, "bench_line": lv, "kernels": {x: recs[x] for x in order}}, f,
+                      indent=1)
         return recs
 
     stats("trace", "kernel_stats_256x128_1MiB", f"Kernel stats: {wl}, encode + recover on one MI355X",

@@ -100,7 +100,9 @@ def test_batch_plan_covers_every_quad_once(tmp_path):
     src = tmp_path / "plan.cpp"
     src.write_text(DRIVER)
     exe = tmp_path / "plan"
-    r = subprocess.run(["g++", "-std=c++17", "-O1", "-I", CSRC, str(src), "-o", str(exe)],
+    # under AddressSanitizer and UndefinedBehaviorSanitizer: every finding aborts
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                        "-I", CSRC, str(src), "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)

@@ -1,4 +1,4 @@
-"""The bindings INTEGRATION.md documents, checked against include/ars.h (the
+"""The bindings of the integration guide at the repository root, checked against include/ars.h (the
 Rust shim cannot be compiled here: no Rust toolchain).  Every function the
 shim's `extern "C"` blocks declare exists in the header with the same
 argument count, argument types and return type; every constant the shim

@@ -52,7 +52,9 @@ def driver(tmp_path_factory):
     src = d / "keyeq.cpp"
     src.write_text(DRIVER)
     exe = d / "keyeq"
-    r = subprocess.run(["g++", "-std=c++17", "-O2", "-I", CSRC, str(src), "-o", str(exe)],
+    # under AddressSanitizer and UndefinedBehaviorSanitizer: every finding aborts
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                        "-I", CSRC, str(src), "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     return d, exe
