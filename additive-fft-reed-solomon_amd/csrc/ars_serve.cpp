@@ -2,7 +2,10 @@
 // continuous batching over the batched persistent kernel.
 //
 // Requests are submitted from any thread.  One dispatcher thread:
-//   1. retires finished batches (event query): statuses, latencies, pool space;
+//   1. retires finished requests (event queries): statuses, latencies, pool
+//      space -- each request at the completion mark recorded after its own
+//      work, so a small request does not wait for a large one launched after
+//      it on the same slot;
 //   2. admits waiting requests in arrival order while the HBM budget (a
 //      device pool holding each admitted request's codeword) has room --
 //      a request that does not fit waits, later smaller ones may pass it;
@@ -97,13 +100,32 @@ struct Req {
 struct Slot {
     ars_batch *batch = nullptr;
     hipStream_t stream = nullptr;
-    hipEvent_t ev = nullptr;
     std::vector<Req *> reqs;
     std::vector<ars_job> jobs;    // run in the batched kernel
     std::vector<ars_job> direct;  // run as plain calls (codewords of direct_min_bytes or more)
+    // Completion marks, recorded on the stream in order: mark m completes
+    // reqs[order[mark_end[m - 1]] .. order[mark_end[m] - 1]] (mark_end[-1] = 0).
+    std::vector<hipEvent_t> evs;  // grown on demand, kept until the server is destroyed
+    std::vector<size_t> mark_end, order;
+    size_t marks = 0, next_mark = 0, next_req = 0;
+    bool synced = false;  // a mark could not be recorded: the stream was synchronised instead
     bool busy = false;
     bool ran_batch = false;  // the last launch used the batched kernel
 };
+
+// Records a completion mark for sl.order[prev .. end); false if no event
+// could be created or recorded.
+bool record_mark(Slot &sl) {
+    if (sl.marks == sl.evs.size()) {
+        hipEvent_t ev;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return false;
+        sl.evs.push_back(ev);
+        sl.mark_end.push_back(0);
+    }
+    if (hipEventRecord(sl.evs[sl.marks], sl.stream) != hipSuccess) return false;
+    sl.mark_end[sl.marks++] = sl.order.size();
+    return true;
+}
 
 }  // namespace
 
@@ -139,10 +161,13 @@ void launch_slot(ars_server *s, Slot &sl, bool loaded) {
     // a request whose input copy cannot be queued fails with that error and
     // stays out of the batch; codewords of direct_min_bytes or more run as
     // plain calls after the batch, on the same stream
-    std::vector<size_t> in_batch, in_direct;
+    std::vector<size_t> in_batch, in_direct, in_failed;
     in_batch.reserve(sl.reqs.size());
     sl.jobs.clear();
     sl.direct.clear();
+    sl.order.clear();
+    sl.marks = sl.next_mark = sl.next_req = 0;
+    sl.synced = false;
     for (size_t i = 0; i < sl.reqs.size(); ++i) {
         Req *q = sl.reqs[i];
         const ars_request &r = q->r;
@@ -167,6 +192,7 @@ void launch_slot(ars_server *s, Slot &sl, bool loaded) {
         }
         if (e != hipSuccess) {
             q->status = ars::hip_error(e, "request input copy");
+            in_failed.push_back(i);
             continue;
         }
         if ((uint64_t)r.n * r.shard_len >= s->cfg.direct_min_bytes) {
@@ -190,12 +216,6 @@ void launch_slot(ars_server *s, Slot &sl, bool loaded) {
         sl.jobs.clear();
     }
     sl.ran_batch = !sl.jobs.empty();
-    const int rc_b = sl.jobs.empty()
-                         ? ARS_OK
-                         : ars_batch_run(sl.batch, sl.jobs.data(), (uint32_t)sl.jobs.size(), 0, sl.stream);
-    const int rc_d = sl.direct.empty()
-                         ? ARS_OK
-                         : ars_run_jobs_plain(s->ctx, sl.direct.data(), (uint32_t)sl.direct.size(), sl.stream);
     auto finish_request = [&](Req *q, int32_t status) {
         const ars_request &r = q->r;
         q->status = status;
@@ -207,13 +227,31 @@ void launch_slot(ars_server *s, Slot &sl, bool loaded) {
         else if (q->status == ARS_TRUE) e = hipMemcpyAsync(r.host_out, d, r.n * L, hipMemcpyDeviceToHost, sl.stream);
         if (e != hipSuccess) q->status = ars::hip_error(e, "request output copy");
     };
-    for (size_t b = 0; b < in_batch.size(); ++b)
-        finish_request(sl.reqs[in_batch[b]], rc_b < 0 ? rc_b : sl.jobs[b].status);
-    for (size_t b = 0; b < in_direct.size(); ++b)
+    // requests whose input copy failed complete with the first mark
+    sl.order.insert(sl.order.end(), in_failed.begin(), in_failed.end());
+    // the batched jobs complete together, then each plain call on its own;
+    // every request's output copy precedes its mark, so its pool space may
+    // be reused as soon as the mark has completed
+    if (!sl.jobs.empty()) {
+        const int rc_b = ars_batch_run(sl.batch, sl.jobs.data(), (uint32_t)sl.jobs.size(), 0, sl.stream);
+        for (size_t b = 0; b < in_batch.size(); ++b) {
+            finish_request(sl.reqs[in_batch[b]], rc_b < 0 ? rc_b : sl.jobs[b].status);
+            sl.order.push_back(in_batch[b]);
+        }
+        (void)record_mark(sl);
+    }
+    for (size_t b = 0; b < in_direct.size(); ++b) {
+        const int rc_d = ars_run_jobs_plain(s->ctx, &sl.direct[b], 1, sl.stream);
         finish_request(sl.reqs[in_direct[b]], rc_d < 0 ? rc_d : sl.direct[b].status);
-    // the slot retires (and its pool space is reused) once the event has
-    // completed; without a recorded event, wait here instead
-    if (hipEventRecord(sl.ev, sl.stream) != hipSuccess) (void)hipStreamSynchronize(sl.stream);
+        sl.order.push_back(in_direct[b]);
+        (void)record_mark(sl);
+    }
+    // requests after the last recorded mark: mark them now, or wait here
+    const size_t marked = sl.marks ? sl.mark_end[sl.marks - 1] : 0;
+    if (marked < sl.order.size() && !record_mark(sl)) {
+        (void)hipStreamSynchronize(sl.stream);
+        sl.synced = true;
+    }
     sl.busy = true;
 }
 
@@ -230,24 +268,33 @@ void finish(ars_server *s, Req *q, Clock::time_point now) {
 
 void dispatcher(ars_server *s) {
     hipSetDevice(s->ctx->device);
-    std::vector<char> done(s->slots.size(), 0);
+    // per slot: the marks found complete by this poll, in order, and how each ended
+    std::vector<size_t> ready(s->slots.size(), 0);
+    std::vector<std::vector<ars::SlotPoll>> polled(s->slots.size());
     // after work, keep polling this long before sleeping on the condition
     // variable: waking a sleeping thread costs tens of microseconds, which a
     // request arriving shortly after the last one would otherwise pay
     const auto linger = std::chrono::microseconds(500);
     Clock::time_point last_work = Clock::now();
-    std::vector<ars::SlotPoll> poll(s->slots.size(), ars::SlotPoll::kRunning);
     for (;;) {
-        // 1. poll the slots' events without the lock (the slots belong to
-        //    this thread), so submitters are not held up meanwhile
+        // 1. poll the slots' marks without the lock (the slots belong to
+        //    this thread), so submitters are not held up meanwhile.  Marks
+        //    complete in stream order: stop at the first one still running.
         bool any_done = false, busy = false;
         for (size_t i = 0; i < s->slots.size(); ++i) {
-            poll[i] = s->slots[i].busy ? ars::classify_slot_poll((int)hipEventQuery(s->slots[i].ev), (int)hipSuccess,
-                                                                 (int)hipErrorNotReady)
-                                       : ars::SlotPoll::kRunning;
-            done[i] = poll[i] != ars::SlotPoll::kRunning;
-            any_done |= done[i] != 0;
-            busy |= s->slots[i].busy;
+            Slot &sl = s->slots[i];
+            polled[i].clear();
+            ready[i] = sl.next_mark;
+            if (!sl.busy) continue;
+            busy = true;
+            while (ready[i] < sl.marks) {
+                const ars::SlotPoll p = ars::classify_slot_poll((int)hipEventQuery(sl.evs[ready[i]]), (int)hipSuccess,
+                                                                (int)hipErrorNotReady);
+                if (p == ars::SlotPoll::kRunning) break;
+                polled[i].push_back(p);
+                ++ready[i];
+            }
+            any_done |= ready[i] > sl.next_mark || (sl.synced && ready[i] == sl.marks);
         }
         if (!any_done && s->pending.load(std::memory_order_acquire) == 0) {
             if (busy || Clock::now() - last_work < linger) {
@@ -264,19 +311,34 @@ void dispatcher(ars_server *s) {
         }
         std::unique_lock<std::mutex> lk(s->mu);
         if (s->pending.exchange(0, std::memory_order_relaxed) != 0 || any_done) last_work = Clock::now();
-        // 2. retire finished batches
+        // 2. retire finished requests
         const Clock::time_point now = Clock::now();
         for (size_t i = 0; i < s->slots.size(); ++i) {
-            if (!done[i]) continue;
             Slot &sl = s->slots[i];
-            for (Req *q : sl.reqs) {
-                q->status = ars::retired_status(poll[i], q->status, ARS_E_HIP);
-                s->pool->release(q->off, q->bytes);
-                finish(s, q, now);
+            if (!sl.busy) continue;
+            auto retire_to = [&](size_t end, ars::SlotPoll p) {
+                for (; sl.next_req < end; ++sl.next_req) {
+                    Req *q = sl.reqs[sl.order[sl.next_req]];
+                    q->status = ars::retired_status(p, q->status, ARS_E_HIP);
+                    s->pool->release(q->off, q->bytes);
+                    finish(s, q, now);
+                }
+            };
+            bool retired = false;
+            for (const ars::SlotPoll p : polled[i]) {
+                retire_to(sl.mark_end[sl.next_mark++], p);
+                retired = true;
             }
-            sl.reqs.clear();
-            sl.busy = false;
-            s->cv_done.notify_all();
+            // past the last mark: the stream was synchronised at launch
+            if (sl.next_mark == sl.marks && sl.synced && sl.next_req < sl.order.size()) {
+                retire_to(sl.order.size(), ars::SlotPoll::kDone);
+                retired = true;
+            }
+            if (sl.next_req == sl.order.size()) {
+                sl.reqs.clear();
+                sl.busy = false;
+            }
+            if (retired) s->cv_done.notify_all();
         }
         // 3. admission against the budget
         for (auto it = s->waiting.begin(); it != s->waiting.end();) {
@@ -342,8 +404,6 @@ int ars_server_create(ars_ctx *ctx, const ars_server_config *cfg, ars_server **o
         int rc = ars_batch_create(ctx, s->cfg.max_batch_jobs, &sl.batch);
         if (rc == ARS_OK && (e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess)
             rc = ars::hip_error(e, "hipStreamCreate");
-        if (rc == ARS_OK && (e = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming)) != hipSuccess)
-            rc = ars::hip_error(e, "hipEventCreate");
         if (rc != ARS_OK) {
             ars_server_destroy(s);
             return rc;
@@ -368,7 +428,7 @@ int ars_server_destroy(ars_server *s) {
     for (Slot &sl : s->slots) {
         if (sl.stream) hipStreamSynchronize(sl.stream);
         if (sl.batch) ars_batch_destroy(sl.batch);
-        if (sl.ev) hipEventDestroy(sl.ev);
+        for (hipEvent_t ev : sl.evs) hipEventDestroy(ev);
         if (sl.stream) hipStreamDestroy(sl.stream);
     }
     for (auto &kv : s->tickets) delete kv.second;
