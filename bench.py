@@ -199,6 +199,55 @@ def probe_valu_rates():
     return {"simple": out[0], "perm": out[1], "butterfly_loop": out[2], "butterfly_straight_line": out[3]}
 
 
+def bench_phaseb(torch):
+    """The bit-sliced phase-B prototype against the product's v_perm form
+    (libars_probe.so, ars_probe_phaseb): 16 rows of 4 MiB, a few maps of phase
+    B (IFFT levels 4..7, high-bit derivative terms, FFT levels 7..4 of the
+    256-point recover) per launch on register-resident data, the bit-sliced
+    form transposing in and out every map as a fused kernel would.  Both
+    outputs are compared; VALU instructions per wave come from the committed
+    rocprofv3 pass (profiles/phaseb_prototype.json) when it was made from
+    these device sources.  None if the probe library is missing."""
+    import ctypes
+    path = os.path.join(ROOT, "additive-fft-reed-solomon_amd", "libars_probe.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    lib.ars_probe_phaseb.restype = ctypes.c_int
+    lib.ars_probe_phaseb.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_double)]
+    maps = lib.ars_probe_phaseb_maps()
+    L = 1 << 22
+    g = torch.Generator(device="cuda")
+    g.manual_seed(49)
+    src = torch.randint(0, 256, (16 * L,), dtype=torch.uint8, device="cuda", generator=g)
+    try:
+        with open(os.path.join(ROOT, "profiles", "phaseb_prototype.json")) as f:
+            doc = json.load(f)
+        prof = doc["kernels"] if doc.get("device_source_sha") == device_source_sha() else {}
+    except (OSError, ValueError, KeyError):
+        prof = {}
+    out = {"workload": f"phase B of the (256,k) recover on 16 rows x {L} columns, {maps} maps per launch",
+           "maps_per_launch": maps}
+    res = []
+    for v, name, kern, cols in ((0, "v_perm_bytes", "{anon}::k_phaseb_bytes", 4),
+                                (1, "bit_sliced", "{anon}::k_phaseb_bits", 32)):
+        o = torch.empty_like(src)
+        us = ctypes.c_double()
+        if lib.ars_probe_phaseb(v, src.data_ptr(), o.data_ptr(), L, 20, ctypes.byref(us)) != 0:
+            return None
+        res.append(o)
+        rec = prof.get(kern)
+        out[name] = {"us": round(us.value, 2), "ps_per_column_map": round(us.value * 1e6 / (L * maps), 3),
+                     "columns_per_lane": cols,
+                     "valu_instr_per_column_map": (round(rec["valu_instr_per_wave"] / (64 * cols * maps), 2)
+                                                   if rec else None)}
+    torch.cuda.synchronize()
+    out["outputs_equal"] = bool(torch.equal(res[0], res[1]))
+    out["bit_sliced_over_v_perm_time"] = round(out["bit_sliced"]["us"] / out["v_perm_bytes"]["us"], 3)
+    return out
+
+
 def valu_roofline(kernel, seconds, rates):
     """Secondary roofline: the kernel's VALU instruction rate (committed
     SQ_INSTS_VALU per wave x waves, profiles/valu_256x128_1MiB.json, over the
@@ -251,6 +300,7 @@ PROFILE_RUNS = {
     "atomic": ("--pmc TCC_ATOMIC TCC_EA0_ATOMIC TCC_EA0_WRREQ TCC_EA0_WRREQ_64B", "--steps 2 --warmup 1 " + PROFILE_HEAD),
     "dram": ("--pmc TCC_EA0_RDREQ_DRAM TCC_EA0_WRREQ_DRAM TCC_EA0_WRREQ_ATOMIC_DRAM TCC_EA0_RDREQ_DRAM_32B",
              "--steps 2 --warmup 1 " + PROFILE_HEAD),
+    "phaseb": ("--pmc SQ_INSTS_VALU SQ_WAVES SQ_BUSY_CYCLES GRBM_GUI_ACTIVE", "--phaseb"),
     "batched": ("--kernel-trace --stats",
                 "--steps 2 --warmup 1 --no-cpu --no-served --no-host --no-decode --no-grid --no-c5 --no-wide"),
     "decode": ("--kernel-trace --stats",
@@ -537,6 +587,45 @@ def summarize_rocprof(d):
     md("lds_256x128_1MiB.md", f"LDS activity per launch: {wl}", [profile_command("lds", "/tmp/prof")],
        "Per-CU figures are chip sums / 256 CUs; LDS busy = SQ_LDS_IDX_ACTIVE per CU over the cycles per XCD "
        "(rocprofiler-sdk's LdsUtil expression for gfx950).\n\n" + t)
+    # the bit-sliced phase-B prototype against the v_perm form (bench_phaseb)
+    if os.path.isdir(os.path.join(d, "phaseb")):
+        pc = ["SQ_INSTS_VALU", "SQ_WAVES", "SQ_BUSY_CYCLES", "GRBM_GUI_ACTIVE"]
+        pk = _by_kernel(_dispatches(os.path.join(d, "phaseb")), pc)
+        try:
+            line = json.loads([x for x in open(os.path.join(d, "phaseb.log")) if x.startswith("{")][-1])
+        except (OSError, IndexError, ValueError):
+            line = None
+        maps = line["maps_per_launch"] if line else 4
+        cols = {"k_phaseb_bytes": 4, "k_phaseb_bits": 32}
+        recs = {}
+        for x, v in sorted(pk.items()):
+            c = next((n for k2, n in cols.items() if x.endswith(k2)), None)
+            if c is None:
+                continue
+            vpw = v["SQ_INSTS_VALU"] / v["SQ_WAVES"]
+            recs[x] = {"dispatches": v["dispatches"], "waves": int(v["SQ_WAVES"]), "valu_instr_per_wave": round(vpw, 1),
+                       "columns_per_lane": c, "valu_instr_per_column_map": round(vpw / (64 * c * maps), 2),
+                       "busy_cycles_per_xcd": int(v["GRBM_GUI_ACTIVE"] / 8)}
+        with open(os.path.join(prof, "phaseb_prototype.json"), "w") as f:
+            json.dump({"workload": "phase B of the 256-point recover, 16 rows x 4 MiB", **stamp, "bench_line": line,
+                       "kernels": recs}, f, indent=1)
+        t = ("| Kernel | Dispatches | Waves | Columns / lane | VALU instr / wave | VALU instr / column / map | "
+             "busy cycles / XCD |\n|---|---|---|---|---|---|---|\n")
+        t += "".join(f"| `{x}` | {v['dispatches']} | {v['waves']} | {v['columns_per_lane']} | {v['valu_instr_per_wave']} | "
+                     f"{v['valu_instr_per_column_map']} | {v['busy_cycles_per_xcd']} |\n" for x, v in recs.items())
+        if line:
+            t += (f"\nThe same run's timing (HIP events, 20 launches): v_perm form "
+                  f"{line['v_perm_bytes']['ps_per_column_map']} ps per column and map, bit-sliced "
+                  f"{line['bit_sliced']['ps_per_column_map']} ps (ratio {line['bit_sliced_over_v_perm_time']}); "
+                  f"outputs equal: {line['outputs_equal']}.\n")
+        md("phaseb_prototype.md", "Bit-sliced phase B against the v_perm form",
+           [profile_command("phaseb", "/tmp/prof")],
+           "Phase B of the 256-point recover (IFFT levels 4..7, the derivative's high-bit terms, FFT levels 7..4; "
+           "compile-time twiddles) on register-resident data, "
+           f"{maps} maps per launch.  `k_phaseb_bytes` is the product's form (one column quad per lane, three "
+           "v_perm_b32 lookups per four byte products); `k_phaseb_bits` holds 32 columns per lane as 8 bit "
+           "planes, multiplies with compile-time XOR networks and transposes each row in and out every map.\n\n"
+           + t)
     print(f"wrote profiles from {d} (device sources {sha[:12]}, commit {commit[:12]})")
 
 
@@ -1170,6 +1259,8 @@ def main():
     ap.add_argument("--summarize-rocprof", metavar="DIR",
                     help="write profiles/ from the rocprofv3 CSV directories under DIR and exit")
     ap.add_argument("--print-source-sha", action="store_true", help="print the device-source SHA and exit")
+    ap.add_argument("--phaseb", action="store_true",
+                    help="run only the bit-sliced phase-B prototype against the v_perm form and print its JSON")
     ap.add_argument("--print-profile-commands", metavar="DIR",
                     help="print the rocprofv3 command of each profile pass, writing under DIR, and exit")
     args = ap.parse_args()
@@ -1182,6 +1273,10 @@ def main():
         return
     if args.summarize_rocprof:
         summarize_rocprof(args.summarize_rocprof)
+        return
+    if args.phaseb:
+        import torch
+        print(json.dumps(bench_phaseb(torch)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); every figure is "
@@ -1288,6 +1383,7 @@ def main():
     traffic = measured_traffic(dom)
     _, stat = _profile_entry("kernel_stats_256x128_1MiB.json", dom)
     valu_rates = probe_valu_rates()
+    phaseb = bench_phaseb(torch)
 
     out = {
         "metric": METRIC,
@@ -1318,6 +1414,7 @@ def main():
                     "frac_hbm": round(rec_gbps / HBM_PEAK_GBPS, 4), "bytes": (n + e) * L,
                     "valu_roofline": valu_roofline(rec_name, s_rec, valu_rates)},
         "autotune": {"encode": tune_enc, "recover": tune_rec},
+        "phaseb_prototype": phaseb,
         "copy_reference": {"GBps": round(copy_gbps, 1), "bytes_read_plus_written": 2 * n * L,
                            "frac_hbm": round(copy_gbps / HBM_PEAK_GBPS, 4)},
         "wall_s_timed_region": round(wall, 4),
