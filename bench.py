@@ -41,14 +41,22 @@ def cpu_info():
     logical CPUs online, CPUs this process may run on, and the flags the
     restatement needs."""
     flags = set()
+    ident = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("flags"):
+            key = line.split(":", 1)[0].strip()
+            if key == "flags":
                 flags = set(line.split(":", 1)[1].split())
-                break
+            elif key in ("cpu family", "model", "stepping", "cache size", "cpu MHz") and key not in ident:
+                ident[key] = line.split(":", 1)[1].strip()
+            if line.strip() == "":
+                break  # the first processor's block is enough
     except OSError:
         pass
-    return {"model": cpu_model(), "nproc_online": os.cpu_count(),
+    # a virtualised host may report only "AMD EPYC" as its model string; the
+    # family / model / stepping numbers still identify the generation
+    return {"model": cpu_model(), "cpuid": {k.replace(" ", "_"): v for k, v in ident.items()},
+            "nproc_online": os.cpu_count(),
             "usable_cpus": len(os.sched_getaffinity(0)),
             "avx512bw": "avx512bw" in flags, "gfni": "gfni" in flags}
 
@@ -473,7 +481,7 @@ def summarize_rocprof(d):
     stats("trace", "kernel_stats_256x128_1MiB", f"Kernel stats: {wl}, encode + recover on one MI355X",
           "The run launches the VALU issue-rate probes (`k_xor`, `k_perm`, `k_butterfly`, `k_butterfly_long`), "
           "the autotuner (every tile variant, 6 launches each), the correctness gate, warm-up, 5 timed steps, "
-          "and max(steps, 10) + 1 launches each of encode alone and recover-with-plan alone. torch kernels are "
+          "and 3 x max(steps, 20) + 1 launches each of encode alone and recover-with-plan alone. torch kernels are "
           "the bench's data set-up, correctness gate and copy reference.", 30)
     if os.path.isdir(os.path.join(d, "batched")):
         stats("batched", "kernel_stats_batched_mixed_4096", "Kernel stats: batched mixed-size workload vs one "
@@ -1346,19 +1354,24 @@ def main():
     plan = torch.empty(ars.lib().ars_recover_plan_size(), dtype=torch.uint8, device=dev)
     ctx.recover_plan(n, k, pos, plan.data_ptr(), stream)
 
-    def timed(fn, reps):
+    def timed(fn, reps, windows=3):
+        """Seconds per launch: the median over `windows` windows of `reps`
+        back-to-back launches, each bracketed by events on the launch stream."""
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         fn()
         torch.cuda.synchronize()
-        a.record(stream)
-        for _ in range(reps):
-            fn()
-        b.record(stream)
-        torch.cuda.synchronize()
-        return a.elapsed_time(b) / reps * 1e-3
+        per = []
+        for _ in range(windows):
+            a.record(stream)
+            for _ in range(reps):
+                fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            per.append(a.elapsed_time(b) / reps * 1e-3)
+        return sorted(per)[len(per) // 2]
 
-    reps = max(args.steps, 10)
+    reps = max(args.steps, 20)
     s_enc = timed(encode, reps)
     s_rec = timed(lambda: ctx.recover_with_plan(n, k, cw.data_ptr(), L, plan.data_ptr(), stream), reps)
     # attainable HBM bandwidth: a plain device-to-device copy of the same
