@@ -53,6 +53,125 @@ __device__ __forceinline__ MulTab lds_tab(const MulTab *t, uint32_t i) {
     return m;
 }
 
+// The v_perm_b32 lookups of K multiplies (K <= 4) issued back to back at wave
+// priority 3, then back to priority 0.  gfx950 issues one wave's v_perm_b32
+// beside other waves' simple VALU ops (XOR, AND, shifts), but with equal
+// priorities the waves of a SIMD, each at its own point of the butterfly,
+// collide: the mix issues ~1.35x slower than its two kinds of instruction
+// apart.  With the lookups in bursts at raised priority the others' simple
+// ops fill in around them, and a register-only loop of these butterflies
+// issues ~1.6x faster.  Inline assembly keeps the lookups together between
+// the two priority changes (registers only, no memory access).
+// s[3k .. 3k+2]: the table indices of multiply k on entry (gmul4's i0, i1,
+// i2), its three partial products on exit.
+template <int K>
+__device__ __forceinline__ void perm_burst(uint32_t (&s)[3 * K], const MulTab (&m)[K]);
+template <>
+__device__ __forceinline__ void perm_burst<1>(uint32_t (&s)[3], const MulTab (&m)[1]) {
+    asm volatile("s_setprio 3\n"
+                 "v_perm_b32 %0, %4, %3, %0\n v_perm_b32 %1, %6, %5, %1\n v_perm_b32 %2, 0, %7, %2\n"
+                 "s_setprio 0\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2])
+                 : "v"(m[0].w[0]), "v"(m[0].w[1]), "v"(m[0].w[2]), "v"(m[0].w[3]), "v"(m[0].w[4]));
+}
+template <>
+__device__ __forceinline__ void perm_burst<2>(uint32_t (&s)[6], const MulTab (&m)[2]) {
+    asm volatile("s_setprio 3\n"
+                 "v_perm_b32 %0, %7, %6, %0\n v_perm_b32 %1, %9, %8, %1\n v_perm_b32 %2, 0, %10, %2\n"
+                 "v_perm_b32 %3, %12, %11, %3\n v_perm_b32 %4, %14, %13, %4\n v_perm_b32 %5, 0, %15, %5\n"
+                 "s_setprio 0\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2]), "+v"(s[3]), "+v"(s[4]), "+v"(s[5])
+                 : "v"(m[0].w[0]), "v"(m[0].w[1]), "v"(m[0].w[2]), "v"(m[0].w[3]), "v"(m[0].w[4]),
+                   "v"(m[1].w[0]), "v"(m[1].w[1]), "v"(m[1].w[2]), "v"(m[1].w[3]), "v"(m[1].w[4]));
+}
+template <>
+__device__ __forceinline__ void perm_burst<3>(uint32_t (&s)[9], const MulTab (&m)[3]) {
+    asm volatile("s_setprio 3\n"
+                 "v_perm_b32 %0, %10, %9, %0\n v_perm_b32 %1, %12, %11, %1\n v_perm_b32 %2, 0, %13, %2\n"
+                 "v_perm_b32 %3, %15, %14, %3\n v_perm_b32 %4, %17, %16, %4\n v_perm_b32 %5, 0, %18, %5\n"
+                 "v_perm_b32 %6, %20, %19, %6\n v_perm_b32 %7, %22, %21, %7\n v_perm_b32 %8, 0, %23, %8\n"
+                 "s_setprio 0\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2]), "+v"(s[3]), "+v"(s[4]), "+v"(s[5]), "+v"(s[6]), "+v"(s[7]),
+                   "+v"(s[8])
+                 : "v"(m[0].w[0]), "v"(m[0].w[1]), "v"(m[0].w[2]), "v"(m[0].w[3]), "v"(m[0].w[4]),
+                   "v"(m[1].w[0]), "v"(m[1].w[1]), "v"(m[1].w[2]), "v"(m[1].w[3]), "v"(m[1].w[4]),
+                   "v"(m[2].w[0]), "v"(m[2].w[1]), "v"(m[2].w[2]), "v"(m[2].w[3]), "v"(m[2].w[4]));
+}
+template <>
+__device__ __forceinline__ void perm_burst<4>(uint32_t (&s)[12], const MulTab (&m)[4]) {
+    asm volatile("s_setprio 3\n"
+                 "v_perm_b32 %0, %13, %12, %0\n v_perm_b32 %1, %15, %14, %1\n v_perm_b32 %2, 0, %16, %2\n"
+                 "v_perm_b32 %3, %18, %17, %3\n v_perm_b32 %4, %20, %19, %4\n v_perm_b32 %5, 0, %21, %5\n"
+                 "v_perm_b32 %6, %23, %22, %6\n v_perm_b32 %7, %25, %24, %7\n v_perm_b32 %8, 0, %26, %8\n"
+                 "v_perm_b32 %9, %28, %27, %9\n v_perm_b32 %10, %30, %29, %10\n v_perm_b32 %11, 0, %31, %11\n"
+                 "s_setprio 0\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2]), "+v"(s[3]), "+v"(s[4]), "+v"(s[5]), "+v"(s[6]), "+v"(s[7]),
+                   "+v"(s[8]), "+v"(s[9]), "+v"(s[10]), "+v"(s[11])
+                 : "v"(m[0].w[0]), "v"(m[0].w[1]), "v"(m[0].w[2]), "v"(m[0].w[3]), "v"(m[0].w[4]),
+                   "v"(m[1].w[0]), "v"(m[1].w[1]), "v"(m[1].w[2]), "v"(m[1].w[3]), "v"(m[1].w[4]),
+                   "v"(m[2].w[0]), "v"(m[2].w[1]), "v"(m[2].w[2]), "v"(m[2].w[3]), "v"(m[2].w[4]),
+                   "v"(m[3].w[0]), "v"(m[3].w[1]), "v"(m[3].w[2]), "v"(m[3].w[3]), "v"(m[3].w[4]));
+}
+
+// Multiplies per lookup burst in a butterfly level.
+constexpr int kBurstMuls = 4;
+
+// The butterflies of a level that multiply (the rest have a compile-time zero
+// twiddle), in register order; see level_pass for the indexing.
+template <int NBF>
+struct MulList {
+    int count;
+    int f[NBF];  // butterfly f: a = x[blk + i], b = x[blk + i + half], blk = (f / half) * 2 * half, i = f % half
+};
+template <int NBF>
+constexpr MulList<NBF> level_muls(int half, int sh, int lv, bool bz) {
+    MulList<NBF> r{};
+    for (int f = 0; f < NBF; ++f) {
+        const int blk = (f / half) * 2 * half;
+        const uint32_t cpart = ((uint32_t)blk << sh) >> (lv + 1);
+        if (!(bz && cpart == 0u)) r.f[r.count++] = f;
+    }
+    return r;
+}
+
+// Multiplies G0 .. G0 + K - 1 of a level (K = min(kBurstMuls, count - G0)):
+// index prep (after INV's b ^= a), one lookup burst, the XOR combines (and
+// FWD's b ^= a); then the next group.
+template <int RL, int SH, int LV, bool INV, bool BZ, int G0>
+__device__ __forceinline__ void level_group(uint32_t *x, uint32_t tbase, const MulTab *tw) {
+    constexpr int half = 1 << (LV - SH);
+    constexpr int NBF = (1 << RL) / 2;
+    constexpr MulList<NBF> ml = level_muls<NBF>(half, SH, LV, BZ);
+    if constexpr (G0 < ml.count) {
+        constexpr int K = (ml.count - G0) < kBurstMuls ? (ml.count - G0) : kBurstMuls;
+        uint32_t s[3 * K];
+        MulTab m[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int f = ml.f[G0 + k];
+            const int blk = (f / half) * 2 * half, i = f % half;
+            uint32_t &a = x[blk + i];
+            uint32_t &b = x[blk + i + half];
+            if constexpr (INV) b ^= a;
+            m[k] = lds_tab(tw, tbase + ((((uint32_t)blk) << SH) >> (LV + 1)));
+            s[3 * k] = b & 0x07070707u;
+            s[3 * k + 1] = (b >> 3) & 0x07070707u;
+            s[3 * k + 2] = (b >> 6) & 0x03030303u;
+        }
+        perm_burst<K>(s, m);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int f = ml.f[G0 + k];
+            const int blk = (f / half) * 2 * half, i = f % half;
+            uint32_t &a = x[blk + i];
+            uint32_t &b = x[blk + i + half];
+            a = xor3(a, s[3 * k], s[3 * k + 1]) ^ s[3 * k + 2];
+            if constexpr (!INV) b ^= a;
+        }
+        level_group<RL, SH, LV, INV, BZ, G0 + K>(x, tbase, tw);
+    }
+}
+
 // One butterfly level over R = 2^RL registers.  Register j holds the point
 // with absolute index  hb + (j << SH) + c,  where hb has zero bits below
 // SH + RL and c < 2^SH never reaches a twiddle (LV >= SH).  The twiddle of
@@ -65,7 +184,8 @@ __device__ __forceinline__ MulTab lds_tab(const MulTab *t, uint32_t i) {
 // BZ = hb is known to be zero: blocks at o = 0 have twiddle s_LV(0) = 0 and
 // become XOR-only at compile time.  Otherwise every block multiplies (the
 // zero twiddle's table is all-zero, so a lane whose block starts at 0 still
-// computes the right value) and no lane-divergent branch is emitted.
+// computes the right value) and no lane-divergent branch is emitted.  The
+// multiplies go in groups of kBurstMuls through perm_burst.
 template <int RL, int SH, int LV, bool INV, bool BZ>
 __device__ __forceinline__ void level_pass(uint32_t *x, uint32_t hb, const MulTab *tw) {
     constexpr int lb = LV - SH;
@@ -78,22 +198,9 @@ __device__ __forceinline__ void level_pass(uint32_t *x, uint32_t hb, const MulTa
         if (BZ && cpart == 0u) {
 #pragma unroll
             for (int i = 0; i < half; ++i) x[blk + i + half] ^= x[blk + i];
-        } else {
-            const MulTab m = lds_tab(tw, tbase + cpart);
-#pragma unroll
-            for (int i = 0; i < half; ++i) {
-                uint32_t &a = x[blk + i];
-                uint32_t &b = x[blk + i + half];
-                if constexpr (INV) {
-                    b ^= a;
-                    a = gmadd4(a, b, m);
-                } else {
-                    a = gmadd4(a, b, m);
-                    b ^= a;
-                }
-            }
         }
     }
+    level_group<RL, SH, LV, INV, BZ, 0>(x, tbase, tw);
 }
 
 // Levels [L0, L1): ascending inverse butterflies (IFFT) or descending
