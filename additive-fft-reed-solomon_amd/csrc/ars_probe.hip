@@ -14,6 +14,8 @@
 //           21 steps x 8 butterflies (~2k VALU instructions, the size of a
 //           fused tile body) with a different table per step, as in a real
 //           transform level
+//   out[4]  out[3]'s butterflies with the lookups in bursts at raised wave
+//           priority (perm_burst), as the tiles issue them
 // Every kernel is pure ALU with fixed trip counts and one store per thread.
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -101,6 +103,52 @@ __global__ void __launch_bounds__(kThreads) k_butterfly_long(uint32_t *out) {
             for (int j = 0; j < 8; ++j) {
                 a[j] = ars::gmadd4(a[j], b[j], m);
                 b[j] ^= a[j];
+            }
+            m.w[0] += 0x01010101u;
+        }
+    }
+    uint32_t r = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r ^= a[j] ^ b[j];
+    out[blockIdx.x * blockDim.x + threadIdx.x] = r;
+}
+
+// k_butterfly_long's butterflies with the table lookups in bursts at raised
+// wave priority, as the tiles issue them (perm_burst: per group of four
+// multiplies, index prep, the twelve v_perm_b32 at priority 3, the combines).
+__global__ void __launch_bounds__(kThreads) k_butterfly_burst(uint32_t *out) {
+    uint32_t a[8], b[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        a[j] = threadIdx.x * 0x01010101u + (uint32_t)j;
+        b[j] = blockIdx.x * 0x00010001u + 3u * (uint32_t)j;
+    }
+    ars::MulTab m;
+    m.w[0] = 0x03020100u + threadIdx.x;
+    m.w[1] = 0x07060504u;
+    m.w[2] = 0x11223344u;
+    m.w[3] = 0x55667788u;
+    m.w[4] = 0x99aabbccu ^ threadIdx.x;
+    m.w[5] = 0u;
+    for (int o = 0; o < kOuter; ++o) {
+#pragma unroll
+        for (int u = 0; u < kSteps; ++u) {
+#pragma unroll
+            for (int g = 0; g < 8; g += 4) {
+                uint32_t sel[12];
+                const ars::MulTab mt[4] = {m, m, m, m};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    sel[3 * k] = b[g + k] & 0x07070707u;
+                    sel[3 * k + 1] = (b[g + k] >> 3) & 0x07070707u;
+                    sel[3 * k + 2] = (b[g + k] >> 6) & 0x03030303u;
+                }
+                ars::perm_burst<4>(sel, mt);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    a[g + k] = ars::xor3(a[g + k], sel[3 * k], sel[3 * k + 1]) ^ sel[3 * k + 2];
+                    b[g + k] ^= a[g + k];
+                }
             }
             m.w[0] += 0x01010101u;
         }
@@ -357,10 +405,11 @@ __global__ void __launch_bounds__(256) k_phaseb_bits(const uint8_t *__restrict__
 // for gfx950 (3 v_perm_b32 and 9 other VALU ops).
 constexpr double kButterflyInstrs = 12.0;
 // k_butterfly_long as compiled for gfx950: 1869 VALU instructions (504
-// v_perm_b32) per outer iteration of 21 steps x 8 butterflies.
+// v_perm_b32) per outer iteration of 21 steps x 8 butterflies; the same for
+// k_butterfly_burst.
 constexpr double kLongStepInstrs = 1869.0 / 21.0;
 
-// out must hold 4 doubles (see the list at the top of this file).
+// out must hold 5 doubles (see the list at the top of this file).
 extern "C" int ars_probe_valu_rates(double *out) {
     if (!out) return -1;
     uint32_t *buf = nullptr;
@@ -369,7 +418,7 @@ extern "C" int ars_probe_valu_rates(double *out) {
     int rc = 0;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) rc = -5;
     const double waves = (double)kBlocks * kThreads / 64.0;
-    for (int k = 0; k < 4 && rc == 0; ++k) {
+    for (int k = 0; k < 5 && rc == 0; ++k) {
         float ms = 0.0f;
         for (int rep = 0; rep < 2 && rc == 0; ++rep) {  // the first launch warms up
             if (hipEventRecord(a) != hipSuccess) rc = -5;
@@ -377,12 +426,13 @@ extern "C" int ars_probe_valu_rates(double *out) {
             if (k == 1) hipLaunchKernelGGL(k_perm, dim3(kBlocks), dim3(kThreads), 0, 0, buf);
             if (k == 2) hipLaunchKernelGGL(k_butterfly, dim3(kBlocks), dim3(kThreads), 0, 0, buf);
             if (k == 3) hipLaunchKernelGGL(k_butterfly_long, dim3(kBlocks), dim3(kThreads), 0, 0, buf);
+            if (k == 4) hipLaunchKernelGGL(k_butterfly_burst, dim3(kBlocks), dim3(kThreads), 0, 0, buf);
             if (hipGetLastError() != hipSuccess || hipEventRecord(b) != hipSuccess ||
                 hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess)
                 rc = -5;
         }
         if (rc == 0) {
-            const double instrs = k == 3 ? waves * kOuter * kSteps * kLongStepInstrs
+            const double instrs = k >= 3 ? waves * kOuter * kSteps * kLongStepInstrs
                                          : waves * kIters * 8.0 * (k == 2 ? kButterflyInstrs : 1.0);
             out[k] = instrs / (ms * 1e-3);
         }

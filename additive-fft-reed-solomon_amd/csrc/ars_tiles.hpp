@@ -91,16 +91,14 @@ __device__ __forceinline__ void body_encode_small(IO &io, uint32_t chunks, const
 // x0 (optional): chunk 0's 16 input rows, already loaded by the caller (the
 // one-codeword kernels issue those loads before staging their tables, so the
 // two latencies overlap).
-// GB_TW: phase B reads its twiddles from gtw, the global copy of the table.
-// Phase B's twiddle indices are the same for every lane, so these become
-// scalar loads instead of LDS reads (each table then needs a v_mov, since a
-// v_perm can take only one SGPR operand); measured 1-3% faster for the
-// 128-point encode, slower for the 256-point recover.
-template <int TL, int PQ, bool ACC, int NBUF, class IO, bool GB_TW = false>
+// Phase B reads its twiddles from the LDS table like phase A.  (Scalar loads
+// from the global table, which every lane indexes alike, were 1-3% faster
+// for the 128-point encode while each v_perm_b32 took its table words where
+// they lay; with the lookups in bursts every table word must sit in a VGPR,
+// and the LDS form measured 1.5% faster.)
+template <int TL, int PQ, bool ACC, int NBUF, class IO>
 __device__ __forceinline__ void body_encode_tile(IO &io, int q, int p, uint32_t *tile, uint32_t chunks,
-                                                 const MulTab *tw, const uint32_t *x0 = nullptr,
-                                                 const MulTab *__restrict__ gtw = nullptr) {
-    const MulTab *twb = GB_TW ? gtw : tw;
+                                                 const MulTab *tw, const uint32_t *x0 = nullptr) {
     using TT = Tile<TL, PQ>;
     constexpr int T = TT::N, Q = TT::Q, GB = TT::GB, RB = TT::RB;
     uint32_t acc[GB][Q], y[GB][Q], x[16];
@@ -120,13 +118,13 @@ __device__ __forceinline__ void body_encode_tile(IO &io, int q, int p, uint32_t 
         TT::get_b(tb, y, q, p);
 #pragma unroll
         for (int m = 0; m < GB; ++m) {
-            run_levels<RB, 4, 4, TL, true, false>(y[m], B, twb);
+            run_levels<RB, 4, 4, TL, true, false>(y[m], B, tw);
 #pragma unroll
             for (int h = 0; h < Q; ++h) acc[m][h] ^= y[m][h];
         }
     }
 #pragma unroll
-    for (int m = 0; m < GB; ++m) run_levels<RB, 4, 4, TL, false, true>(acc[m], 0u, twb);
+    for (int m = 0; m < GB; ++m) run_levels<RB, 4, 4, TL, false, true>(acc[m], 0u, tw);
     // No barrier before put_b, even with one buffer: put_b writes exactly the
     // words this lane's last get_b read (same index set), and one lane's LDS
     // read and later write of a word are ordered.

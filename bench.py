@@ -201,10 +201,11 @@ def probe_valu_rates():
     lib = ctypes.CDLL(path)
     lib.ars_probe_valu_rates.restype = ctypes.c_int
     lib.ars_probe_valu_rates.argtypes = [ctypes.POINTER(ctypes.c_double)]
-    out = (ctypes.c_double * 4)()
+    out = (ctypes.c_double * 5)()
     if lib.ars_probe_valu_rates(out) != 0:
         return None
-    return {"simple": out[0], "perm": out[1], "butterfly_loop": out[2], "butterfly_straight_line": out[3]}
+    return {"simple": out[0], "perm": out[1], "butterfly_loop": out[2], "butterfly_straight_line": out[3],
+            "butterfly_burst": out[4]}
 
 
 def bench_phaseb(torch):
@@ -260,8 +261,11 @@ def valu_roofline(kernel, seconds, rates):
     """Secondary roofline: the kernel's VALU instruction rate (committed
     SQ_INSTS_VALU per wave x waves, profiles/valu_256x128_1MiB.json, over the
     launch time measured now) against the attainable rate for its instruction
-    mix: the live simple-op and v_perm issue rates weighted by the kernel's
-    v_perm share."""
+    mix.  gfx950 issues one wave's v_perm_b32 beside other waves' simple ops,
+    so the peak is the overlapped bound: the simple ops at the live v_xor_b32
+    rate or the lookups at the live v_perm_b32 rate, whichever takes longer.
+    The kernel is also set against the register-only probes of its butterfly:
+    default wave priorities, and the lookup bursts the tiles use."""
     try:
         with open(os.path.join(ROOT, "profiles", "valu_256x128_1MiB.json")) as f:
             perm = json.load(f)["perm_fraction"]
@@ -274,14 +278,15 @@ def valu_roofline(kernel, seconds, rates):
     if fam is None:
         return None
     achieved = rec["valu_instr_per_wave"] * rec["waves"] / seconds
-    peak = 1.0 / (fam / rates["perm"] + (1.0 - fam) / rates["simple"])
+    peak = 1.0 / max(fam / rates["perm"], (1.0 - fam) / rates["simple"])
     return {"bound": "valu", "kernel": name, "achieved": round(achieved / 1e12, 3), "peak": round(peak / 1e12, 3),
             "unit": "T wave64-instr/s", "frac": round(achieved / peak, 4), "perm_fraction": fam,
-            "butterfly_loop_frac": round(rates["butterfly_loop"] / peak, 4),
             # the kernel against the same multiply-add butterflies issued in
             # straight-line code (its instruction mix is that butterfly's)
             "frac_of_straight_line_butterfly_rate": round(achieved / rates["butterfly_straight_line"], 4),
-            "peak_from": "live v_xor_b32 and v_perm_b32 issue rates weighted by the kernel's v_perm share"}
+            "frac_of_burst_butterfly_rate": round(achieved / rates["butterfly_burst"], 4),
+            "peak_from": "live v_xor_b32 and v_perm_b32 issue rates, overlapped: max(simple share / v_xor rate, "
+                         "v_perm share / v_perm rate)"}
 
 
 # ---------------------------------------------------------------------------
@@ -408,8 +413,9 @@ def _perm_fractions(kernels):
         body = next((b for sym, b in bodies.items() if sym.startswith(_mangle(k))), None)
         if body is None:
             continue
-        valu = sum(1 for l in body if re.match(r"^\s+v_", l))
-        perm = sum(1 for l in body if re.match(r"^\s+v_perm_b32", l))
+        # (inline-assembly lines may start at column 0)
+        valu = sum(1 for l in body if re.match(r"^\s*v_", l))
+        perm = sum(1 for l in body if re.match(r"^\s*v_perm_b32", l))
         if valu:
             out[k] = {"value": round(perm / valu, 4),
                       "source": f"{perm} v_perm_b32 of {valu} VALU instructions in the gfx950 assembly of {k}"}
@@ -577,8 +583,9 @@ def summarize_rocprof(d):
     md("valu_256x128_1MiB.md", f"VALU issue: {wl}", [profile_command("valu", "/tmp/prof")],
        "SQ_INSTS_VALU counts wave64 VALU instructions over the device; GRBM_GUI_ACTIVE is summed over the 8 "
        "XCDs (busy cycles per XCD = value / 8).  The v_perm share is counted in the kernel's gfx950 assembly "
-       "(each GF(2^8) multiply of four bytes is three v_perm_b32 table lookups); bench.py weights the live "
-       "v_xor_b32 and v_perm_b32 issue rates by it for the VALU roofline.\n\n" + t)
+       "(each GF(2^8) multiply of four bytes is three v_perm_b32 table lookups, counted in the inline-assembly "
+       "bursts too); bench.py's VALU roofline takes the overlapped bound of the live v_xor_b32 and v_perm_b32 "
+       "issue rates for that share.\n\n" + t)
     lc = ["SQ_LDS_IDX_ACTIVE", "SQ_LDS_BANK_CONFLICT", "SQ_LDS_DATA_FIFO_FULL", "SQ_LDS_CMD_FIFO_FULL", "SQ_INSTS_LDS",
           "SQ_WAVES", "SQ_WAIT_INST_LDS", "SQ_ACTIVE_INST_LDS", "GRBM_GUI_ACTIVE"]
     lk = _by_kernel(_dispatches(os.path.join(d, "lds")), lc)
