@@ -113,7 +113,8 @@ __device__ __forceinline__ void perm_burst<4>(uint32_t (&s)[12], const MulTab (&
                    "v"(m[3].w[0]), "v"(m[3].w[1]), "v"(m[3].w[2]), "v"(m[3].w[3]), "v"(m[3].w[4]));
 }
 
-// Multiplies per lookup burst in a butterfly level.
+// Multiplies per lookup burst in a butterfly level (the default; two where a
+// kernel has no registers to spare).
 constexpr int kBurstMuls = 4;
 
 // The butterflies of a level that multiply (the rest have a compile-time zero
@@ -134,16 +135,16 @@ constexpr MulList<NBF> level_muls(int half, int sh, int lv, bool bz) {
     return r;
 }
 
-// Multiplies G0 .. G0 + K - 1 of a level (K = min(kBurstMuls, count - G0)):
+// Multiplies G0 .. G0 + K - 1 of a level (K = min(GRP, count - G0)):
 // index prep (after INV's b ^= a), one lookup burst, the XOR combines (and
 // FWD's b ^= a); then the next group.
-template <int RL, int SH, int LV, bool INV, bool BZ, int G0>
+template <int RL, int SH, int LV, bool INV, bool BZ, int GRP, int G0>
 __device__ __forceinline__ void level_group(uint32_t *x, uint32_t tbase, const MulTab *tw) {
     constexpr int half = 1 << (LV - SH);
     constexpr int NBF = (1 << RL) / 2;
     constexpr MulList<NBF> ml = level_muls<NBF>(half, SH, LV, BZ);
     if constexpr (G0 < ml.count) {
-        constexpr int K = (ml.count - G0) < kBurstMuls ? (ml.count - G0) : kBurstMuls;
+        constexpr int K = (ml.count - G0) < GRP ? (ml.count - G0) : GRP;
         uint32_t s[3 * K];
         MulTab m[K];
 #pragma unroll
@@ -168,7 +169,7 @@ __device__ __forceinline__ void level_group(uint32_t *x, uint32_t tbase, const M
             a = xor3(a, s[3 * k], s[3 * k + 1]) ^ s[3 * k + 2];
             if constexpr (!INV) b ^= a;
         }
-        level_group<RL, SH, LV, INV, BZ, G0 + K>(x, tbase, tw);
+        level_group<RL, SH, LV, INV, BZ, GRP, G0 + K>(x, tbase, tw);
     }
 }
 
@@ -185,8 +186,8 @@ __device__ __forceinline__ void level_group(uint32_t *x, uint32_t tbase, const M
 // become XOR-only at compile time.  Otherwise every block multiplies (the
 // zero twiddle's table is all-zero, so a lane whose block starts at 0 still
 // computes the right value) and no lane-divergent branch is emitted.  The
-// multiplies go in groups of kBurstMuls through perm_burst.
-template <int RL, int SH, int LV, bool INV, bool BZ>
+// multiplies go in groups of GRP through perm_burst.
+template <int RL, int SH, int LV, bool INV, bool BZ, int GRP = kBurstMuls>
 __device__ __forceinline__ void level_pass(uint32_t *x, uint32_t hb, const MulTab *tw) {
     constexpr int lb = LV - SH;
     static_assert(lb >= 0 && lb < RL, "level outside the register range");
@@ -200,20 +201,20 @@ __device__ __forceinline__ void level_pass(uint32_t *x, uint32_t hb, const MulTa
             for (int i = 0; i < half; ++i) x[blk + i + half] ^= x[blk + i];
         }
     }
-    level_group<RL, SH, LV, INV, BZ, 0>(x, tbase, tw);
+    level_group<RL, SH, LV, INV, BZ, GRP, 0>(x, tbase, tw);
 }
 
 // Levels [L0, L1): ascending inverse butterflies (IFFT) or descending
 // forward butterflies (FFT).
-template <int RL, int SH, int L0, int L1, bool INV, bool BZ>
+template <int RL, int SH, int L0, int L1, bool INV, bool BZ, int GRP = kBurstMuls>
 __device__ __forceinline__ void run_levels(uint32_t *x, uint32_t hb, const MulTab *tw) {
     if constexpr (L0 < L1) {
         if constexpr (INV) {
-            level_pass<RL, SH, L0, true, BZ>(x, hb, tw);
-            run_levels<RL, SH, L0 + 1, L1, true, BZ>(x, hb, tw);
+            level_pass<RL, SH, L0, true, BZ, GRP>(x, hb, tw);
+            run_levels<RL, SH, L0 + 1, L1, true, BZ, GRP>(x, hb, tw);
         } else {
-            level_pass<RL, SH, L1 - 1, false, BZ>(x, hb, tw);
-            run_levels<RL, SH, L0, L1 - 1, false, BZ>(x, hb, tw);
+            level_pass<RL, SH, L1 - 1, false, BZ, GRP>(x, hb, tw);
+            run_levels<RL, SH, L0, L1 - 1, false, BZ, GRP>(x, hb, tw);
         }
     }
 }

@@ -226,9 +226,13 @@ __device__ __forceinline__ void recover_stream(uint8_t *data, uint32_t L, uint32
     recover_rows_in<OOB>(io, f, q, rows);
     for (;;) {
         const uint32_t tn = t + gridDim.x;
+        // wide rows keep their row addresses on the scalar unit but need two more
+        // VGPRs; with four multiplies per lookup burst the 256-point body then
+        // spills 8 bytes, with two it fits
+        constexpr int GRP = (AM == 3 && NL >= 8) ? 2 : kBurstMuls;
         // the next tile's addressing is formed only where its rows are
         // requested, so it is not live across phase B
-        body_recover_tile1<NL, PQ>(io, tabs, f, q, p, tile, tw, rows, [&] {
+        body_recover_tile1<NL, PQ, GRP>(io, tabs, f, q, p, tile, tw, rows, [&] {
             if (tn < tiles)
                 recover_rows_in<OOB>(make_bufio<AM, UNI>(data, N * L, data, N * L, tn * PQ + p, L), f, q, rows);
         });
@@ -520,9 +524,12 @@ static hipError_t rec_fused_one(uint8_t *data, uint32_t L, const PosArg &pos, ui
         const uint32_t grid = ceil_div(quads, 256);
         if (grid > max_groups) return hipErrorNotSupported;
         hipLaunchKernelGGL((k_recover_small_fused<NL, AM>), dim3(grid), dim3(256), 0, s, data, L, pos, e, gtab, tw);
+    } else if constexpr (AM == 3) {
+        // a codeword of 4 GiB or more has at least 2^22 column quads, far more
+        // than one workgroup per compute unit covers
+        return hipErrorNotSupported;
     } else if constexpr (AM != 0) {
         constexpr int PQ = default_pq(1, NL);
-        static_assert(AM != 3 || PQ % 64 == 0, "wide rows need one shard group per wave");
         const uint32_t grid = ceil_div(quads, PQ);
         if (grid > max_groups) return hipErrorNotSupported;
         if (oob_ok(NL, AM, L))

@@ -390,7 +390,8 @@ __device__ __forceinline__ void recover_rows_in(const IO &io, const Flags16 &f, 
     }
 }
 
-template <int NL, int PQ, class IO, class TABS, class AFTER>
+// GRP: multiplies per lookup burst (see level_pass).
+template <int NL, int PQ, int GRP, class IO, class TABS, class AFTER>
 __device__ __forceinline__ void body_recover_tile1(const IO &io, const TABS &tabs, const Flags16 &f, int q, int p,
                                                    uint32_t *tile, const MulTab *tw, const uint32_t (&rows)[16],
                                                    AFTER &&after_exchange) {
@@ -407,14 +408,14 @@ __device__ __forceinline__ void body_recover_tile1(const IO &io, const TABS &tab
         else
             x[j] = gmul4(rows[j], tabs.tab(q * 16 + j));
     }
-    run_levels<4, 0, 0, 4, true, false>(x, q * 16, tw);
+    run_levels<4, 0, 0, 4, true, false, GRP>(x, q * 16, tw);
     low_derivative(x, dlo);
     TT::put_a(tile, x, q, p);
     __syncthreads();
     TT::get_b(tile, y, q, p);
 #pragma unroll
     for (int m = 0; m < GB; ++m) {
-        run_levels<RB, 4, 4, NL, true, true>(y[m], 0u, tw);
+        run_levels<RB, 4, 4, NL, true, true, GRP>(y[m], 0u, tw);
         // Dhi in place: ascending h only reads higher, still untouched, indices
 #pragma unroll
         for (int h = 0; h < Q; ++h) {
@@ -424,7 +425,7 @@ __device__ __forceinline__ void body_recover_tile1(const IO &io, const TABS &tab
                 if (!((h >> b) & 1)) a ^= y[m][h | (1 << b)];
             y[m][h] = a;
         }
-        run_levels<RB, 4, 4, NL, false, true>(y[m], 0u, tw);
+        run_levels<RB, 4, 4, NL, false, true, GRP>(y[m], 0u, tw);
     }
     // put_b rewrites exactly the words this lane's get_b read
     TT::put_b(tile, y, q, p);
@@ -433,7 +434,7 @@ __device__ __forceinline__ void body_recover_tile1(const IO &io, const TABS &tab
     after_exchange();
 #pragma unroll
     for (int j = 0; j < 16; ++j) x[j] ^= dlo[j];
-    run_levels<4, 0, 0, 4, false, false>(x, q * 16, tw);
+    run_levels<4, 0, 0, 4, false, false, GRP>(x, q * 16, tw);
 #pragma unroll
     for (int j = 0; j < 16; ++j)
         if (f.at(j)) io.xor_out((uint32_t)(q * 16), j, gmul4(x[j], tabs.tab(q * 16 + j)));

@@ -80,16 +80,19 @@ def test_tile_kernels_address_lds_directly(asm):
 def test_default_kernels_do_not_spill(asm):
     # the variants the launchers pick without tuning: PQ 32 / NBUF 1 for the
     # 128-point encode, PQ 64 / NBUF 2 below; recover: the streaming kernel with
-    # 64 quads (either load form), and on the byte path PQ 32 / NBUF 2 at 256
-    # points, PQ 64 / NBUF 2 below; the one-launch recovers use the same variants.
+    # 32 quads at 256 points and 64 below (either load form; 64 quads for wide
+    # rows), and on the byte path PQ 32 / NBUF 2 at 256 points, PQ 64 / NBUF 2
+    # below; the one-launch recovers use the same variants.
     # A spill matters doubly in the streaming kernel: a scratch reload waits for
     # every memory operation issued before it, the prefetched rows included.
     # (encode and the streaming recover in access modes 1 and 2: aligned dwords
     # and dwords shifted past alignment)
     want = [r"k_encode_tileILi7ELi32ELi[12]ELb0ELi1ELb1E", r"k_encode_tileILi6ELi64ELi[12]ELb0ELi2E",
             r"k_recover_streamILi8ELi64ELb1ELi[12]E", r"k_recover_streamILi8ELi64ELb0ELi[12]E",
+            r"k_recover_streamILi8ELi32ELb1ELi[12]E", r"k_recover_streamILi8ELi32ELb0ELi[12]E",
+            r"k_recover_stream_fusedILi8ELi32ELb1ELi[12]E", r"k_recover_streamILi8ELi64ELb[01]ELi3E",
             r"k_recover_streamILi7ELi64ELb1ELi[12]E", r"k_recover_streamILi5ELi64ELb1ELi[12]E",
-            r"k_recover_stream_fusedILi8ELi64ELb1ELi[12]E", r"k_recover_tileILi8ELi32ELb0ELi2E",
+            r"k_recover_tileILi8ELi32ELb0ELi2E",
             r"k_recover_tile_fusedILi8ELi32ELb0ELi2E", r"k_recover_tile_fusedILi5ELi64ELb0ELi2E",
             r"k_recover_small_fusedILi4ELi[012]E", r"k_syndrome_tile", r"k_decode_columns"]
     seen = {}
