@@ -113,6 +113,88 @@ __device__ __forceinline__ void perm_burst<4>(uint32_t (&s)[12], const MulTab (&
                    "v"(m[3].w[0]), "v"(m[3].w[1]), "v"(m[3].w[2]), "v"(m[3].w[3]), "v"(m[3].w[4]));
 }
 
+// The same lookups for tables read at a wave-uniform index (see UniTw): each
+// table's high words and its T2 word are scalar operands (one SGPR per
+// v_perm_b32 is allowed), only the two low words occupy VGPRs.
+template <int K>
+__device__ __forceinline__ void perm_burst_u(uint32_t (&s)[3 * K], const MulTab (&m)[K]);
+template <>
+__device__ __forceinline__ void perm_burst_u<1>(uint32_t (&s)[3], const MulTab (&m)[1]) {
+    asm volatile("s_setprio 3\n"
+                 "v_perm_b32 %0, %4, %3, %0\n v_perm_b32 %1, %6, %5, %1\n v_perm_b32 %2, 0, %7, %2\n"
+                 "s_setprio 0\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2])
+                 : "v"(m[0].w[0]), "s"(m[0].w[1]), "v"(m[0].w[2]), "s"(m[0].w[3]), "s"(m[0].w[4]));
+}
+template <>
+__device__ __forceinline__ void perm_burst_u<2>(uint32_t (&s)[6], const MulTab (&m)[2]) {
+    asm volatile("s_setprio 3\n"
+                 "v_perm_b32 %0, %7, %6, %0\n v_perm_b32 %1, %9, %8, %1\n v_perm_b32 %2, 0, %10, %2\n"
+                 "v_perm_b32 %3, %12, %11, %3\n v_perm_b32 %4, %14, %13, %4\n v_perm_b32 %5, 0, %15, %5\n"
+                 "s_setprio 0\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2]), "+v"(s[3]), "+v"(s[4]), "+v"(s[5])
+                 : "v"(m[0].w[0]), "s"(m[0].w[1]), "v"(m[0].w[2]), "s"(m[0].w[3]), "s"(m[0].w[4]),
+                   "v"(m[1].w[0]), "s"(m[1].w[1]), "v"(m[1].w[2]), "s"(m[1].w[3]), "s"(m[1].w[4]));
+}
+template <>
+__device__ __forceinline__ void perm_burst_u<3>(uint32_t (&s)[9], const MulTab (&m)[3]) {
+    asm volatile("s_setprio 3\n"
+                 "v_perm_b32 %0, %10, %9, %0\n v_perm_b32 %1, %12, %11, %1\n v_perm_b32 %2, 0, %13, %2\n"
+                 "v_perm_b32 %3, %15, %14, %3\n v_perm_b32 %4, %17, %16, %4\n v_perm_b32 %5, 0, %18, %5\n"
+                 "v_perm_b32 %6, %20, %19, %6\n v_perm_b32 %7, %22, %21, %7\n v_perm_b32 %8, 0, %23, %8\n"
+                 "s_setprio 0\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2]), "+v"(s[3]), "+v"(s[4]), "+v"(s[5]), "+v"(s[6]), "+v"(s[7]),
+                   "+v"(s[8])
+                 : "v"(m[0].w[0]), "s"(m[0].w[1]), "v"(m[0].w[2]), "s"(m[0].w[3]), "s"(m[0].w[4]),
+                   "v"(m[1].w[0]), "s"(m[1].w[1]), "v"(m[1].w[2]), "s"(m[1].w[3]), "s"(m[1].w[4]),
+                   "v"(m[2].w[0]), "s"(m[2].w[1]), "v"(m[2].w[2]), "s"(m[2].w[3]), "s"(m[2].w[4]));
+}
+template <>
+__device__ __forceinline__ void perm_burst_u<4>(uint32_t (&s)[12], const MulTab (&m)[4]) {
+    asm volatile("s_setprio 3\n"
+                 "v_perm_b32 %0, %13, %12, %0\n v_perm_b32 %1, %15, %14, %1\n v_perm_b32 %2, 0, %16, %2\n"
+                 "v_perm_b32 %3, %18, %17, %3\n v_perm_b32 %4, %20, %19, %4\n v_perm_b32 %5, 0, %21, %5\n"
+                 "v_perm_b32 %6, %23, %22, %6\n v_perm_b32 %7, %25, %24, %7\n v_perm_b32 %8, 0, %26, %8\n"
+                 "v_perm_b32 %9, %28, %27, %9\n v_perm_b32 %10, %30, %29, %10\n v_perm_b32 %11, 0, %31, %11\n"
+                 "s_setprio 0\n"
+                 : "+v"(s[0]), "+v"(s[1]), "+v"(s[2]), "+v"(s[3]), "+v"(s[4]), "+v"(s[5]), "+v"(s[6]), "+v"(s[7]),
+                   "+v"(s[8]), "+v"(s[9]), "+v"(s[10]), "+v"(s[11])
+                 : "v"(m[0].w[0]), "s"(m[0].w[1]), "v"(m[0].w[2]), "s"(m[0].w[3]), "s"(m[0].w[4]),
+                   "v"(m[1].w[0]), "s"(m[1].w[1]), "v"(m[1].w[2]), "s"(m[1].w[3]), "s"(m[1].w[4]),
+                   "v"(m[2].w[0]), "s"(m[2].w[1]), "v"(m[2].w[2]), "s"(m[2].w[3]), "s"(m[2].w[4]),
+                   "v"(m[3].w[0]), "s"(m[3].w[1]), "v"(m[3].w[2]), "s"(m[3].w[3]), "s"(m[3].w[4]));
+}
+
+// Where a butterfly level takes its twiddle tables from.
+// LdsTw: the workgroup's LDS copy, at any (lane-varying) index.
+struct LdsTw {
+    const MulTab *t;
+    static constexpr bool kUniform = false;
+    __device__ __forceinline__ MulTab get(uint32_t i) const { return lds_tab(t, i); }
+};
+// UniTw: an index that is the same in every lane of the wave.  The two low
+// words come from the LDS copy (8 bytes per lane instead of 20), the high
+// words and T2 by scalar loads from the same table in global memory (t and g
+// hold equal contents).  Only for wave-uniform indices: the high words enter
+// perm_burst_u as scalar operands, so a lane-varying index would be wrong.
+struct UniTw {
+    const MulTab *t;
+    const MulTab *__restrict__ g;
+    static constexpr bool kUniform = true;
+    __device__ __forceinline__ MulTab get(uint32_t i) const {
+        const uint32_t *lp = reinterpret_cast<const uint32_t *>(t + i);
+        const uint32_t *gp = reinterpret_cast<const uint32_t *>(g + i);
+        MulTab m;
+        m.w[0] = lp[0];
+        m.w[2] = lp[2];
+        m.w[1] = gp[1];
+        m.w[3] = gp[3];
+        m.w[4] = gp[4];
+        m.w[5] = 0u;
+        return m;
+    }
+};
+
 // Multiplies per lookup burst in a butterfly level (the default; two where a
 // kernel has no registers to spare).
 constexpr int kBurstMuls = 4;
@@ -138,8 +220,8 @@ constexpr MulList<NBF> level_muls(int half, int sh, int lv, bool bz) {
 // Multiplies G0 .. G0 + K - 1 of a level (K = min(GRP, count - G0)):
 // index prep (after INV's b ^= a), one lookup burst, the XOR combines (and
 // FWD's b ^= a); then the next group.
-template <int RL, int SH, int LV, bool INV, bool BZ, int GRP, int G0>
-__device__ __forceinline__ void level_group(uint32_t *x, uint32_t tbase, const MulTab *tw) {
+template <int RL, int SH, int LV, bool INV, bool BZ, int GRP, int G0, class TS>
+__device__ __forceinline__ void level_group(uint32_t *x, uint32_t tbase, const TS &tw) {
     constexpr int half = 1 << (LV - SH);
     constexpr int NBF = (1 << RL) / 2;
     constexpr MulList<NBF> ml = level_muls<NBF>(half, SH, LV, BZ);
@@ -154,12 +236,15 @@ __device__ __forceinline__ void level_group(uint32_t *x, uint32_t tbase, const M
             uint32_t &a = x[blk + i];
             uint32_t &b = x[blk + i + half];
             if constexpr (INV) b ^= a;
-            m[k] = lds_tab(tw, tbase + ((((uint32_t)blk) << SH) >> (LV + 1)));
+            m[k] = tw.get(tbase + ((((uint32_t)blk) << SH) >> (LV + 1)));
             s[3 * k] = b & 0x07070707u;
             s[3 * k + 1] = (b >> 3) & 0x07070707u;
             s[3 * k + 2] = (b >> 6) & 0x03030303u;
         }
-        perm_burst<K>(s, m);
+        if constexpr (TS::kUniform)
+            perm_burst_u<K>(s, m);
+        else
+            perm_burst<K>(s, m);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int f = ml.f[G0 + k];
@@ -169,7 +254,7 @@ __device__ __forceinline__ void level_group(uint32_t *x, uint32_t tbase, const M
             a = xor3(a, s[3 * k], s[3 * k + 1]) ^ s[3 * k + 2];
             if constexpr (!INV) b ^= a;
         }
-        level_group<RL, SH, LV, INV, BZ, GRP, G0 + K>(x, tbase, tw);
+        level_group<RL, SH, LV, INV, BZ, GRP, G0 + K, TS>(x, tbase, tw);
     }
 }
 
@@ -187,8 +272,8 @@ __device__ __forceinline__ void level_group(uint32_t *x, uint32_t tbase, const M
 // zero twiddle's table is all-zero, so a lane whose block starts at 0 still
 // computes the right value) and no lane-divergent branch is emitted.  The
 // multiplies go in groups of GRP through perm_burst.
-template <int RL, int SH, int LV, bool INV, bool BZ, int GRP = kBurstMuls>
-__device__ __forceinline__ void level_pass(uint32_t *x, uint32_t hb, const MulTab *tw) {
+template <int RL, int SH, int LV, bool INV, bool BZ, int GRP = kBurstMuls, class TS = LdsTw>
+__device__ __forceinline__ void level_pass(uint32_t *x, uint32_t hb, const TS &tw) {
     constexpr int lb = LV - SH;
     static_assert(lb >= 0 && lb < RL, "level outside the register range");
     constexpr int half = 1 << lb;
@@ -201,22 +286,27 @@ __device__ __forceinline__ void level_pass(uint32_t *x, uint32_t hb, const MulTa
             for (int i = 0; i < half; ++i) x[blk + i + half] ^= x[blk + i];
         }
     }
-    level_group<RL, SH, LV, INV, BZ, GRP, 0>(x, tbase, tw);
+    level_group<RL, SH, LV, INV, BZ, GRP, 0, TS>(x, tbase, tw);
 }
 
 // Levels [L0, L1): ascending inverse butterflies (IFFT) or descending
 // forward butterflies (FFT).
-template <int RL, int SH, int L0, int L1, bool INV, bool BZ, int GRP = kBurstMuls>
-__device__ __forceinline__ void run_levels(uint32_t *x, uint32_t hb, const MulTab *tw) {
+template <int RL, int SH, int L0, int L1, bool INV, bool BZ, int GRP = kBurstMuls, class TS = LdsTw>
+__device__ __forceinline__ void run_levels_t(uint32_t *x, uint32_t hb, const TS &tw) {
     if constexpr (L0 < L1) {
         if constexpr (INV) {
-            level_pass<RL, SH, L0, true, BZ, GRP>(x, hb, tw);
-            run_levels<RL, SH, L0 + 1, L1, true, BZ, GRP>(x, hb, tw);
+            level_pass<RL, SH, L0, true, BZ, GRP, TS>(x, hb, tw);
+            run_levels_t<RL, SH, L0 + 1, L1, true, BZ, GRP, TS>(x, hb, tw);
         } else {
-            level_pass<RL, SH, L1 - 1, false, BZ, GRP>(x, hb, tw);
-            run_levels<RL, SH, L0, L1 - 1, false, BZ, GRP>(x, hb, tw);
+            level_pass<RL, SH, L1 - 1, false, BZ, GRP, TS>(x, hb, tw);
+            run_levels_t<RL, SH, L0, L1 - 1, false, BZ, GRP, TS>(x, hb, tw);
         }
     }
+}
+// ... with the tables from the workgroup's LDS copy at any index
+template <int RL, int SH, int L0, int L1, bool INV, bool BZ, int GRP = kBurstMuls>
+__device__ __forceinline__ void run_levels(uint32_t *x, uint32_t hb, const MulTab *tw) {
+    run_levels_t<RL, SH, L0, L1, INV, BZ, GRP>(x, hb, LdsTw{tw});
 }
 
 // Shard rows of one buffer, addressed as (per-lane byte offset, uniform

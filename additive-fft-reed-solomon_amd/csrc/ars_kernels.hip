@@ -84,7 +84,9 @@ __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
     for (int j = 0; j < 16; ++j) x0[j] = io.ld_in((uint32_t)(q * 16), j);
     stage_twiddles<TT::THREADS>(tw, g_tw);
     __syncthreads();
-    body_encode_tile<TL, PQ, ACC, NBUF, BufIO<AM>>(io, q, p, tile, ONE ? 1u : chunks, tw, x0);
+    // with one shard group per wave the twiddle indices are wave-uniform:
+    // high table words by scalar loads (see enc_levels)
+    body_encode_tile<TL, PQ, ACC, NBUF, BufIO<AM>, (PQ % 64 == 0)>(io, q, p, tile, ONE ? 1u : chunks, tw, x0, g_tw);
 }
 
 template <int KL, int AM>

@@ -23,15 +23,16 @@ constexpr int kGtabBytes = 1024;
 
 // Default tile variant (quads per workgroup, LDS buffers) of an operation
 // (0 = encode, 1 = recover) on 2^tl points, used when the context holds no
-// tuned choice.  Encode: 32 quads for the 128-point transform with one
-// buffer, 64 quads with two below.  Recover: the streaming kernel
-// (k_recover_stream, one buffer) with 32 quads at 256 points and 64 below.
-// With 64 quads a wave holds one shard group and skips erased rows with
-// scalar branches; at 256 points the 32-quad tile, two workgroups per CU and
-// no such branches, measured 3.5% faster once the butterflies' lookups went
-// into priority bursts.  The byte path has only the two-buffer kernel.
-constexpr int default_pq(int op, int tl) { return ((op == 0 && tl >= 7) || (op == 1 && tl >= 8)) ? 32 : 64; }
-constexpr int default_nbuf(int op, int tl) { return op == 1 ? 1 : (tl >= 7 ? 1 : 2); }
+// tuned choice.  Encode: 64 quads with one buffer, where a wave holds one
+// shard group and its butterflies take the tables' high words by scalar
+// loads (fastest or within 2% of it at every T from 32 to 128).  Recover:
+// the streaming kernel (k_recover_stream, one buffer) with 32 quads at 256
+// points and 64 below.  With 64 quads a wave skips erased rows with scalar
+// branches; at 256 points the 32-quad tile, two workgroups per CU and no
+// such branches, measured 3.5% faster once the butterflies' lookups went into
+// priority bursts.  The byte path has only the two-buffer kernel.
+constexpr int default_pq(int op, int tl) { return (op == 1 && tl >= 8) ? 32 : 64; }
+constexpr int default_nbuf(int op, int tl) { return (void)op, (void)tl, 1; }
 
 // pq / nbuf select a tile variant for transforms of 32+ points (0 = default):
 // pq = column quads per workgroup (16, 32 or 64), nbuf = LDS exchange buffers (1 or 2).

@@ -78,8 +78,8 @@ def test_tile_kernels_address_lds_directly(asm):
 
 
 def test_default_kernels_do_not_spill(asm):
-    # the variants the launchers pick without tuning: PQ 32 / NBUF 1 for the
-    # 128-point encode, PQ 64 / NBUF 2 below; recover: the streaming kernel with
+    # the variants the launchers pick without tuning: PQ 64 / NBUF 1 for
+    # encode (all four access modes); recover: the streaming kernel with
     # 32 quads at 256 points and 64 below (either load form; 64 quads for wide
     # rows), and on the byte path PQ 32 / NBUF 2 at 256 points, PQ 64 / NBUF 2
     # below; the one-launch recovers use the same variants.
@@ -87,7 +87,7 @@ def test_default_kernels_do_not_spill(asm):
     # every memory operation issued before it, the prefetched rows included.
     # (encode and the streaming recover in access modes 1 and 2: aligned dwords
     # and dwords shifted past alignment)
-    want = [r"k_encode_tileILi7ELi32ELi[12]ELb0ELi1ELb1E", r"k_encode_tileILi6ELi64ELi[12]ELb0ELi2E",
+    want = [r"k_encode_tileILi7ELi64ELi[0123]ELb[01]ELi1ELb[01]E", r"k_encode_tileILi[56]ELi64ELi[0123]ELb[01]ELi1E",
             r"k_recover_streamILi8ELi64ELb1ELi[12]E", r"k_recover_streamILi8ELi64ELb0ELi[12]E",
             r"k_recover_streamILi8ELi32ELb1ELi[12]E", r"k_recover_streamILi8ELi32ELb0ELi[12]E",
             r"k_recover_stream_fusedILi8ELi32ELb1ELi[12]E", r"k_recover_streamILi8ELi64ELb[01]ELi3E",
