@@ -306,8 +306,10 @@ int ars_plain_replay(ars_ctx *ctx, const ars_request *reqs, const double *arriva
  * (1: half the LDS, extra barriers; 2: one barrier per exchange, and for
  * recover the formal derivative's low-bit terms taken without an LDS copy).
  * ars_autotune times every variant for one (n, k, shard_len) on scratch
- * buffers and keeps the fastest in the context for that operation and
- * transform size.  ars_batch_autotune picks the persistent grid (workgroups
+ * buffers (three interleaved rounds, the fastest round of each) and keeps the
+ * fastest in the context for that operation and transform size, or the
+ * default variant when none beats it by more than 2%.
+ * ars_batch_autotune picks the persistent grid (workgroups
  * per compute unit) for a representative batch.
  * ------------------------------------------------------------------------ */
 typedef struct ars_tune_result {
@@ -315,8 +317,8 @@ typedef struct ars_tune_result {
     uint32_t log2_size;   /* transform size: T for encode, n for recover */
     uint32_t quads;       /* chosen column quads per workgroup (0: not tunable) */
     uint32_t buffers;     /* chosen LDS exchange buffers */
-    double best_us;       /* mean kernel time of the chosen variant */
-    double default_us;    /* mean kernel time of the default variant */
+    double best_us;       /* kernel time of the chosen variant (mean of 5 launches, fastest round) */
+    double default_us;    /* the same for the default variant */
 } ars_tune_result;
 
 int ars_autotune(ars_ctx *ctx, uint32_t n, uint32_t k, size_t shard_len, ars_tune_result *encode_out,
