@@ -138,8 +138,8 @@ __device__ __forceinline__ BJob uni(const BJob &x) {
 // Tile body of variant V (see batch_region) for transform size 2^TL.
 template <int TL, int V, class IO>
 __device__ __forceinline__ void run_body(const BItem &it, IO &io, int q, int p, uint32_t *tile,
-                                         const MulTab *tw, const MulTab *s_tab, const uint8_t *s_er,
-                                         uint32_t si) {
+                                         const MulTab *tw, const MulTab *gtw, const MulTab *s_tab,
+                                         const uint8_t *s_er, uint32_t si) {
     constexpr uint32_t PQ = batch_lanes(TL);
     if constexpr (TL <= 4) {
         if constexpr (V == 0 || V == 1) {
@@ -151,7 +151,10 @@ __device__ __forceinline__ void run_body(const BItem &it, IO &io, int q, int p, 
             body_recover_small<TL>(io, tabs, tw);
         }
     } else if constexpr (V == 0 || V == 1) {
-        if constexpr (TL <= 7) body_encode_tile<TL, PQ, V == 1, 1>(io, q, p, tile, it.arg, tw);
+        // 64+ quads: a wave holds one shard group, so the tables' high words
+        // come by scalar loads (see enc_levels)
+        if constexpr (TL <= 7)
+            body_encode_tile<TL, PQ, V == 1, 1, IO, (PQ % 64 == 0)>(io, q, p, tile, it.arg, tw, nullptr, gtw);
     } else if constexpr (V == 2) {
         if constexpr (TL <= 7) body_encode_kpow2_tile<TL, PQ, 1>(io, q, p, tile, it.arg & 0xffffu, it.arg >> 16, tw);
     } else {
@@ -170,18 +173,18 @@ __device__ __forceinline__ void run_body(const BItem &it, IO &io, int q, int p, 
 // item's base pointers).
 template <int TL, int V, int PACK>
 __device__ __forceinline__ void run_tile(const BItem &it, const BSeg *segs, const BJob *jobs, uint32_t r,
-                                         uint32_t *tile, const MulTab *tw, const MulTab *s_tab,
-                                         const uint8_t *s_er) {
+                                         uint32_t *tile, const MulTab *tw, const MulTab *gtw,
+                                         const MulTab *s_tab, const uint8_t *s_er) {
     constexpr uint32_t PQ = batch_lanes(TL);
     const int p = (int)(threadIdx.x % PQ);
-    const int q = (int)(threadIdx.x / PQ);
+    const int q = PQ % 64 == 0 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / PQ)) : (int)(threadIdx.x / PQ);
     if constexpr (PACK == 0) {
         const uint32_t jid = uni(segs[it.seg0].job), quad0 = uni(segs[it.seg0].quad0);
         const BJob job = uni(jobs[jid]);
         const uint32_t quad = quad0 + (uint32_t)p + r * PQ;
         BufIO<true> io = make_bufio<true>(job.in, job.in_bytes, job.out, job.out_bytes,
                                           quad < job.quads ? quad : job.quads, job.L);
-        run_body<TL, V>(it, io, q, p, tile, tw, s_tab, s_er, 0u);
+        run_body<TL, V>(it, io, q, p, tile, tw, gtw, s_tab, s_er, 0u);
     } else if constexpr (TL <= kBatchPackMaxTl) {
         BJob job;
         uint32_t col, nb, si;
@@ -193,7 +196,7 @@ __device__ __forceinline__ void run_tile(const BItem &it, const BSeg *segs, cons
         io.vout = (uint32_t)(job.out - it.out_base) + col;
         io.L = job.L;
         io.nb = nb;
-        run_body<TL, V>(it, io, q, p, tile, tw, s_tab, s_er, si);
+        run_body<TL, V>(it, io, q, p, tile, tw, gtw, s_tab, s_er, si);
     }
 }
 
@@ -202,8 +205,8 @@ __device__ __forceinline__ void run_tile(const BItem &it, const BSeg *segs, cons
 // item's LDS reads (tile, plans) before this item's writes.
 template <int TL, int V, int PACK>
 __device__ __forceinline__ void run_region(BHeader *h, const BItem *items, const BSeg *segs, const BJob *jobs,
-                                           uint32_t *tile, const MulTab *tw, MulTab *s_tab, uint8_t *s_er,
-                                           uint32_t *s_item) {
+                                           uint32_t *tile, const MulTab *tw, const MulTab *gtw, MulTab *s_tab,
+                                           uint8_t *s_er, uint32_t *s_item) {
     constexpr uint32_t g = batch_region(TL, V, PACK);
     const uint32_t begin = uni(h->begin[g]), end = uni(h->end[g]);
     if (begin >= end) return;
@@ -221,7 +224,7 @@ __device__ __forceinline__ void run_region(BHeader *h, const BItem *items, const
         const uint32_t rep = it.nseg_rep >> 16;
         for (uint32_t r = 0; r < rep; ++r) {
             if (r) __syncthreads();  // the previous tile's LDS reads are done
-            run_tile<TL, V, PACK>(it, segs, jobs, r, tile, tw, s_tab, s_er);
+            run_tile<TL, V, PACK>(it, segs, jobs, r, tile, tw, gtw, s_tab, s_er);
         }
     }
 }
@@ -229,22 +232,22 @@ __device__ __forceinline__ void run_region(BHeader *h, const BItem *items, const
 // Every region of transform size 2^TL, then the next smaller size.
 template <int TL>
 __device__ __forceinline__ void run_sizes(BHeader *h, const BItem *items, const BSeg *segs, const BJob *jobs,
-                                          uint32_t *tile, const MulTab *tw, MulTab *s_tab, uint8_t *s_er,
-                                          uint32_t *s_item) {
+                                          uint32_t *tile, const MulTab *tw, const MulTab *gtw, MulTab *s_tab,
+                                          uint8_t *s_er, uint32_t *s_item) {
     if constexpr (TL >= 0) {
         if constexpr (TL <= 7) {  // encode transforms are at most 128 points
-            run_region<TL, 0, 0>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
-            run_region<TL, 1, 0>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
-            run_region<TL, 2, 0>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
+            run_region<TL, 0, 0>(h, items, segs, jobs, tile, tw, gtw, s_tab, s_er, s_item);
+            run_region<TL, 1, 0>(h, items, segs, jobs, tile, tw, gtw, s_tab, s_er, s_item);
+            run_region<TL, 2, 0>(h, items, segs, jobs, tile, tw, gtw, s_tab, s_er, s_item);
         }
-        if constexpr (TL >= 1) run_region<TL, 3, 0>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
+        if constexpr (TL >= 1) run_region<TL, 3, 0>(h, items, segs, jobs, tile, tw, gtw, s_tab, s_er, s_item);
         if constexpr (TL <= kBatchPackMaxTl) {
-            run_region<TL, 0, 1>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
-            run_region<TL, 1, 1>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
-            run_region<TL, 2, 1>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
-            if constexpr (TL >= 1) run_region<TL, 3, 1>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
+            run_region<TL, 0, 1>(h, items, segs, jobs, tile, tw, gtw, s_tab, s_er, s_item);
+            run_region<TL, 1, 1>(h, items, segs, jobs, tile, tw, gtw, s_tab, s_er, s_item);
+            run_region<TL, 2, 1>(h, items, segs, jobs, tile, tw, gtw, s_tab, s_er, s_item);
+            if constexpr (TL >= 1) run_region<TL, 3, 1>(h, items, segs, jobs, tile, tw, gtw, s_tab, s_er, s_item);
         }
-        run_sizes<TL - 1>(h, items, segs, jobs, tile, tw, s_tab, s_er, s_item);
+        run_sizes<TL - 1>(h, items, segs, jobs, tile, tw, gtw, s_tab, s_er, s_item);
     }
 }
 
@@ -265,7 +268,7 @@ __global__ void __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per
         for (uint32_t i = threadIdx.x; i < (uint32_t)kTwiddleEntries * 6u; i += blockDim.x) dst[i] = src[i];
     }
     __syncthreads();
-    run_sizes<8>(h, items, segs, jobs, tile, tw, s_tab, s_er, &s_item);
+    run_sizes<8>(h, items, segs, jobs, tile, tw, g_tw, s_tab, s_er, &s_item);
 }
 
 hipError_t launch_batch_reset(uint8_t *arena, hipStream_t s) {
