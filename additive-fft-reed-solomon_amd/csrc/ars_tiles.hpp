@@ -365,8 +365,11 @@ __device__ __forceinline__ void body_recover_tile(IO &io, const TABS &tabs, int 
 // one-codeword kernels load the next tile's rows into `rows` there.  The caller synchronises before reusing
 // `tile` (other lanes' get_a reads of it).
 // Bit j: row q*16 + j is erased (one VGPR; the 16 flag bytes would take four).
+// all / none: every row of every shard group this wave holds is erased / none
+// is (wave-uniform, from a ballot).
 struct Flags16 {
     uint32_t m;
+    bool all, none;
     __device__ __forceinline__ bool at(int j) const { return (m >> j) & 1u; }
 };
 
@@ -382,7 +385,9 @@ __device__ __forceinline__ Flags16 group_flags(const TABS &tabs, int q) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) m |= ((w[j >> 2] >> (8 * (j & 3))) & 0xffu) ? (1u << j) : 0u;
     if constexpr (UNI) m = __builtin_amdgcn_readfirstlane(m);
-    return Flags16{m};
+    const bool all = __builtin_amdgcn_ballot_w64(m != 0xffffu) == 0;
+    const bool none = __builtin_amdgcn_ballot_w64(m != 0u) == 0;
+    return Flags16{m, all, none};
 }
 
 // OOB: erased rows are read at io.oob, past the end of the buffer, where a
@@ -413,18 +418,25 @@ __device__ __forceinline__ void body_recover_tile1(const IO &io, const TABS &tab
     using TT = Tile<NL, PQ>;
     constexpr int Q = TT::Q, GB = TT::GB, RB = TT::RB;
     uint32_t x[16], y[GB][Q], dlo[16];
-    // Erased rows are 0 and stay 0.  With a wave-uniform mask (PQ >= 64) their
-    // multiply is skipped; otherwise it is done (a divergent branch here costs
-    // registers).
+    if (f.all) {
+        // every input row of the wave is erased, i.e. 0: so is its phase-A
+        // IFFT and that output's derivative terms
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        if (PQ % 64 == 0 && f.at(j))
-            x[j] = 0u;
-        else
-            x[j] = gmul4(rows[j], tabs.tab(q * 16 + j));
+        for (int j = 0; j < 16; ++j) x[j] = dlo[j] = 0u;
+    } else {
+        // Erased rows are 0 and stay 0.  With a wave-uniform mask (PQ >= 64)
+        // their multiply is skipped; otherwise it is done (a divergent branch
+        // here costs registers).
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if (PQ % 64 == 0 && f.at(j))
+                x[j] = 0u;
+            else
+                x[j] = gmul4(rows[j], tabs.tab(q * 16 + j));
+        }
+        run_levels<4, 0, 0, 4, true, false, GRP>(x, q * 16, tw);
+        low_derivative(x, dlo);
     }
-    run_levels<4, 0, 0, 4, true, false, GRP>(x, q * 16, tw);
-    low_derivative(x, dlo);
     TT::put_a(tile, x, q, p);
     __syncthreads();
     TT::get_b(tile, y, q, p);
@@ -447,6 +459,8 @@ __device__ __forceinline__ void body_recover_tile1(const IO &io, const TABS &tab
     __syncthreads();
     TT::get_a(tile, x, q, p);
     after_exchange();
+    // no output row in the wave: its phase-A FFT is not needed
+    if (f.none) return;
 #pragma unroll
     for (int j = 0; j < 16; ++j) x[j] ^= dlo[j];
     run_levels<4, 0, 0, 4, false, false, GRP>(x, q * 16, tw);

@@ -113,3 +113,36 @@ def test_65536_tiny_codewords_batched_equals_plain(ars):
     assert all(s == 1 for s in st)
     assert torch.equal(cws, ref)
     assert b.info()["items"] < count // 8
+
+
+@pytest.mark.parametrize("n,k", [(256, 128), (128, 64)])
+def test_whole_wave_patterns_round_trip(ars, n, k):
+    # erasure patterns where whole waves of the streaming recover hold only
+    # erased rows, or none (a wave covers 32 rows at n = 256, 16 below): those
+    # waves skip their phase-A IFFT or FFT; 256 KiB shards run the streaming
+    # kernel with more than one workgroup per CU, the garbage-free round trip
+    # must give back the codeword
+    L = 256 << 10
+    T = n - k
+    w = 32 if n == 256 else 16
+    g = torch.Generator(device="cuda")
+    g.manual_seed(47)
+    cw = torch.empty(n * L, dtype=torch.uint8, device="cuda")
+    cw[T * L:] = torch.randint(0, 256, (k * L,), dtype=torch.uint8, device="cuda", generator=g)
+    codec = ars.Codec(n, k)
+    codec.encode_systematic_sharded(cw[T * L:], cw[:T * L], None, L, accumulate=False)
+    ref = cw.clone()
+    pats = {
+        "first_half": list(range(T)),
+        "alternate_waves": [r for r in range(n) if (r // w) % 2 == 0][:T],
+        "waves_plus_partial": (list(range(0, w)) + list(range(3 * w, 3 * w + w // 2)) +
+                               list(range(n - w // 2, n)))[:T],
+    }
+    for name, pos in pats.items():
+        v = cw.view(n, L)
+        v[torch.tensor(pos, dtype=torch.int64, device="cuda")] = 0
+        assert codec.recover_erasure_shards(cw, None, L, pos)
+        torch.cuda.synchronize()
+        assert torch.equal(cw, ref), (n, k, name)
+    del ref, cw
+    torch.cuda.empty_cache()
