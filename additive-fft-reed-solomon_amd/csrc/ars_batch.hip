@@ -152,7 +152,7 @@ __device__ __forceinline__ void run_body(const BItem &it, IO &io, int q, int p, 
         }
     } else if constexpr (V == 0 || V == 1) {
         // 64+ quads: a wave holds one shard group, so the tables' high words
-        // come by scalar loads (see enc_levels)
+        // come by scalar loads (see uni_levels)
         if constexpr (TL <= 7)
             body_encode_tile<TL, PQ, V == 1, 1, IO, (PQ % 64 == 0)>(io, q, p, tile, it.arg, tw, nullptr, gtw);
     } else if constexpr (V == 2) {

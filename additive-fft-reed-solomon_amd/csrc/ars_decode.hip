@@ -108,7 +108,11 @@ __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
     stage_tw(tw, g_tw);
     if (threadIdx.x < PQ) s_or[threadIdx.x] = 0u;
     __syncthreads();
-    const int p = threadIdx.x % PQ, q = threadIdx.x / PQ;
+    // with 64 quads a wave holds one shard group: wave-uniform twiddle indices,
+    // the tables' high words by scalar loads (uni_levels)
+    constexpr bool UT = PQ % 64 == 0;
+    const int p = threadIdx.x % PQ;
+    const int q = UT ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x / PQ)) : (int)(threadIdx.x / PQ);
     const uint32_t quad = blockIdx.x * PQ + p;
     BufIO<AL> io = make_bufio<AL>(rx, chunks * T * L, const_cast<uint8_t *>(rx), 0u, quad, L);
     uint32_t acc[GB][Q], y[GB][Q], x[16];
@@ -120,7 +124,7 @@ __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
         const uint32_t B = c * T;
 #pragma unroll
         for (int j = 0; j < 16; ++j) x[j] = io.ld_in(B + q * 16, j);
-        run_levels<4, 0, 0, 4, true, false>(x, B + q * 16, tw);
+        uni_levels<4, 0, 0, 4, true, false, UT>(x, B + q * 16, tw, g_tw);
         uint32_t *tb = tile + (NBUF == 2 ? (c & 1) * TT::WORDS : 0);
         if (NBUF == 1 && c > 0) __syncthreads();
         TT::put_a(tb, x, q, p);
@@ -130,10 +134,10 @@ __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
         // zero twiddle, XOR-only at compile time
         if (c == 0) {
 #pragma unroll
-            for (int m = 0; m < GB; ++m) run_levels<RB, 4, 4, TL, true, true>(y[m], 0u, tw);
+            for (int m = 0; m < GB; ++m) uni_levels<RB, 4, 4, TL, true, true, UT>(y[m], 0u, tw, g_tw);
         } else {
 #pragma unroll
-            for (int m = 0; m < GB; ++m) run_levels<RB, 4, 4, TL, true, false>(y[m], B, tw);
+            for (int m = 0; m < GB; ++m) uni_levels<RB, 4, 4, TL, true, false, UT>(y[m], B, tw, g_tw);
         }
 #pragma unroll
         for (int m = 0; m < GB; ++m)
@@ -489,9 +493,9 @@ hipError_t syn_one(const uint8_t *rx, uint32_t L, uint32_t chunks, const MulTab 
         hipLaunchKernelGGL((k_syndrome_small<TL, AL>), dim3(ceil_div(quads, 256)), dim3(256), 0, s, rx, L, chunks,
                            tw, so);
     } else {
-        // the encode tile's measured best shapes: 32 quads and one LDS buffer
-        // at 128 points, 64 quads and two buffers below
-        constexpr int PQ = TL >= 7 ? 32 : 64, NB = TL >= 7 ? 1 : 2;
+        // the encode tile's default shape: 64 quads (scalar table words) and
+        // one LDS buffer
+        constexpr int PQ = 64, NB = 1;
         hipLaunchKernelGGL((k_syndrome_tile<TL, PQ, NB, AL>), dim3(ceil_div(quads, PQ)),
                            dim3(Tile<TL, PQ>::THREADS), 0, s, rx, L, chunks, tw, so);
     }

@@ -85,7 +85,7 @@ __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
     stage_twiddles<TT::THREADS>(tw, g_tw);
     __syncthreads();
     // with one shard group per wave the twiddle indices are wave-uniform:
-    // high table words by scalar loads (see enc_levels)
+    // high table words by scalar loads (see uni_levels)
     body_encode_tile<TL, PQ, ACC, NBUF, BufIO<AM>, (PQ % 64 == 0)>(io, q, p, tile, ONE ? 1u : chunks, tw, x0, g_tw);
 }
 

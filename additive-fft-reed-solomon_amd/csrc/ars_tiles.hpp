@@ -102,7 +102,7 @@ __device__ __forceinline__ void body_encode_small(IO &io, uint32_t chunks, const
 // each lane reads 8 of a table's 20 bytes from LDS.  The LDS traffic is then
 // mostly the two exchanges (measured 8.7% faster for the 128-point encode).
 template <int RL, int SH, int L0, int L1, bool INV, bool BZ, bool UT>
-__device__ __forceinline__ void enc_levels(uint32_t *x, uint32_t hb, const MulTab *tw, const MulTab *gtw) {
+__device__ __forceinline__ void uni_levels(uint32_t *x, uint32_t hb, const MulTab *tw, const MulTab *gtw) {
     if constexpr (UT)
         run_levels_t<RL, SH, L0, L1, INV, BZ, kBurstMuls, UniTw>(x, hb, UniTw{tw, gtw});
     else
@@ -125,7 +125,7 @@ __device__ __forceinline__ void body_encode_tile(IO &io, int q, int p, uint32_t 
         const uint32_t B = (c + 1) * T;
 #pragma unroll
         for (int j = 0; j < 16; ++j) x[j] = (x0 && c == 0) ? x0[j] : io.ld_in(c * T + q * 16, j);
-        enc_levels<4, 0, 0, 4, true, false, UT>(x, B + q * 16, tw, gtw);
+        uni_levels<4, 0, 0, 4, true, false, UT>(x, B + q * 16, tw, gtw);
         uint32_t *tb = tile + (NBUF == 2 ? (c & 1) * TT::WORDS : 0);
         if (NBUF == 1 && c > 0) __syncthreads();
         TT::put_a(tb, x, q, p);
@@ -133,13 +133,13 @@ __device__ __forceinline__ void body_encode_tile(IO &io, int q, int p, uint32_t 
         TT::get_b(tb, y, q, p);
 #pragma unroll
         for (int m = 0; m < GB; ++m) {
-            enc_levels<RB, 4, 4, TL, true, false, UT>(y[m], B, tw, gtw);
+            uni_levels<RB, 4, 4, TL, true, false, UT>(y[m], B, tw, gtw);
 #pragma unroll
             for (int h = 0; h < Q; ++h) acc[m][h] ^= y[m][h];
         }
     }
 #pragma unroll
-    for (int m = 0; m < GB; ++m) enc_levels<RB, 4, 4, TL, false, true, UT>(acc[m], 0u, tw, gtw);
+    for (int m = 0; m < GB; ++m) uni_levels<RB, 4, 4, TL, false, true, UT>(acc[m], 0u, tw, gtw);
     // No barrier before put_b, even with one buffer: put_b writes exactly the
     // words this lane's last get_b read (same index set), and one lane's LDS
     // read and later write of a word are ordered.
@@ -147,7 +147,7 @@ __device__ __forceinline__ void body_encode_tile(IO &io, int q, int p, uint32_t 
     TT::put_b(tb, acc, q, p);
     __syncthreads();
     TT::get_a(tb, x, q, p);
-    enc_levels<4, 0, 0, 4, false, false, UT>(x, q * 16, tw, gtw);
+    uni_levels<4, 0, 0, 4, false, false, UT>(x, q * 16, tw, gtw);
 #pragma unroll
     for (int j = 0; j < 16; ++j) io.st_out((uint32_t)(q * 16), j, x[j]);
 }
