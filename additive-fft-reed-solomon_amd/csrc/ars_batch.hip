@@ -156,7 +156,9 @@ __device__ __forceinline__ void run_body(const BItem &it, IO &io, int q, int p, 
         if constexpr (TL <= 7)
             body_encode_tile<TL, PQ, V == 1, 1, IO, (PQ % 64 == 0)>(io, q, p, tile, it.arg, tw, nullptr, gtw);
     } else if constexpr (V == 2) {
-        if constexpr (TL <= 7) body_encode_kpow2_tile<TL, PQ, 1>(io, q, p, tile, it.arg & 0xffffu, it.arg >> 16, tw);
+        if constexpr (TL <= 7)
+            body_encode_kpow2_tile<TL, PQ, 1, IO, (PQ % 64 == 0)>(io, q, p, tile, it.arg & 0xffffu, it.arg >> 16, tw,
+                                                                  gtw);
     } else {
         // The one-buffer streaming body (body_recover_tile1) measured slower
         // here: 32-quad items leave the erased flags lane-varying, and this

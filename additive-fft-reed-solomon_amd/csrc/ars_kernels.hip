@@ -113,7 +113,7 @@ __global__ void __launch_bounds__((Tile<KL, PQ>::THREADS))
     __syncthreads();
     const int p = threadIdx.x % PQ, q = tile_q<PQ>();
     BufIO<AM> io = make_bufio<AM>(msg, TT::N * L, par, blocks * TT::N * L, blockIdx.x * PQ + p, L);
-    body_encode_kpow2_tile<KL, PQ, 2>(io, q, p, tile, blocks, msg_base, tw);
+    body_encode_kpow2_tile<KL, PQ, 2, BufIO<AM>, (PQ % 64 == 0)>(io, q, p, tile, blocks, msg_base, tw, g_tw);
 }
 
 __global__ void __launch_bounds__(256) k_recover_prep(RecPrep *__restrict__ out, uint32_t n,
@@ -303,7 +303,7 @@ __global__ void __launch_bounds__((Tile<TL, PQ>::THREADS))
     __syncthreads();
     const int p = threadIdx.x % PQ, q = tile_q<PQ>();
     BufIO<AM> io = make_bufio<AM>(data, 2 * TT::N * L, data, 2 * TT::N * L, blockIdx.x * PQ + p, L);
-    body_recompute_tile<TL, PQ>(io, q, p, tile, tw);
+    body_recompute_tile<TL, PQ, BufIO<AM>, (PQ % 64 == 0)>(io, q, p, tile, tw, g_tw);
 }
 
 // ---------------------------------------------------------------------------

@@ -172,34 +172,36 @@ __device__ __forceinline__ void body_encode_kpow2_small(IO &io, uint32_t blocks,
     }
 }
 
-template <int KL, int PQ, int NBUF, class IO>
+// UT: scalar table words, as body_encode_tile.
+template <int KL, int PQ, int NBUF, class IO, bool UT = false>
 __device__ __forceinline__ void body_encode_kpow2_tile(IO &io, int q, int p, uint32_t *tile,
                                                        uint32_t blocks, uint32_t msg_base,
-                                                       const MulTab *tw) {
+                                                       const MulTab *tw, const MulTab *gtw = nullptr) {
+    static_assert(!UT || PQ % 64 == 0, "scalar table words need one shard group per wave");
     using TT = Tile<KL, PQ>;
     constexpr int K = TT::N, Q = TT::Q, GB = TT::GB, RB = TT::RB;
     uint32_t x[16], d[GB][Q], y[GB][Q];
 #pragma unroll
     for (int j = 0; j < 16; ++j) x[j] = io.ld_in((uint32_t)(q * 16), j);
-    run_levels<4, 0, 0, 4, true, false>(x, msg_base + q * 16, tw);
+    uni_levels<4, 0, 0, 4, true, false, UT>(x, msg_base + q * 16, tw, gtw);
     TT::put_a(tile, x, q, p);
     __syncthreads();
     TT::get_b(tile, d, q, p);
 #pragma unroll
-    for (int m = 0; m < GB; ++m) run_levels<RB, 4, 4, KL, true, false>(d[m], msg_base, tw);
+    for (int m = 0; m < GB; ++m) uni_levels<RB, 4, 4, KL, true, false, UT>(d[m], msg_base, tw, gtw);
     for (uint32_t b = 0; b < blocks; ++b) {
 #pragma unroll
         for (int m = 0; m < GB; ++m) {
 #pragma unroll
             for (int h = 0; h < Q; ++h) y[m][h] = d[m][h];
-            run_levels<RB, 4, 4, KL, false, false>(y[m], b * K, tw);
+            uni_levels<RB, 4, 4, KL, false, false, UT>(y[m], b * K, tw, gtw);
         }
         uint32_t *tb = tile + (NBUF == 2 ? ((b + 1) & 1) * TT::WORDS : 0);
         if (NBUF == 1) __syncthreads();
         TT::put_b(tb, y, q, p);
         __syncthreads();
         TT::get_a(tb, x, q, p);
-        run_levels<4, 0, 0, 4, false, false>(x, b * K + q * 16, tw);
+        uni_levels<4, 0, 0, 4, false, false, UT>(x, b * K + q * 16, tw, gtw);
 #pragma unroll
         for (int j = 0; j < 16; ++j) io.st_out(b * K + q * 16, j, x[j]);
     }
@@ -485,27 +487,30 @@ __device__ __forceinline__ void body_recompute_small(IO &io, const MulTab *tw) {
     for (int j = 0; j < T; ++j) io.st_out((uint32_t)T, j, x[j]);
 }
 
-template <int TL, int PQ, class IO>
-__device__ __forceinline__ void body_recompute_tile(IO &io, int q, int p, uint32_t *tile, const MulTab *tw) {
+// UT: scalar table words, as body_encode_tile.
+template <int TL, int PQ, class IO, bool UT = false>
+__device__ __forceinline__ void body_recompute_tile(IO &io, int q, int p, uint32_t *tile, const MulTab *tw,
+                                                    const MulTab *gtw = nullptr) {
+    static_assert(!UT || PQ % 64 == 0, "scalar table words need one shard group per wave");
     using TT = Tile<TL, PQ>;
     constexpr int T = TT::N, Q = TT::Q, GB = TT::GB, RB = TT::RB;
     uint32_t x[16], y[GB][Q];
 #pragma unroll
     for (int j = 0; j < 16; ++j) x[j] = io.ld_in((uint32_t)(q * 16), j);
-    run_levels<4, 0, 0, 4, true, false>(x, q * 16, tw);
+    uni_levels<4, 0, 0, 4, true, false, UT>(x, q * 16, tw, gtw);
     TT::put_a(tile, x, q, p);
     __syncthreads();
     TT::get_b(tile, y, q, p);
 #pragma unroll
     for (int m = 0; m < GB; ++m) {
-        run_levels<RB, 4, 4, TL, true, true>(y[m], 0u, tw);
-        run_levels<RB, 4, 4, TL, false, false>(y[m], (uint32_t)T, tw);
+        uni_levels<RB, 4, 4, TL, true, true, UT>(y[m], 0u, tw, gtw);
+        uni_levels<RB, 4, 4, TL, false, false, UT>(y[m], (uint32_t)T, tw, gtw);
     }
     uint32_t *t1 = tile + TT::WORDS;
     TT::put_b(t1, y, q, p);
     __syncthreads();
     TT::get_a(t1, x, q, p);
-    run_levels<4, 0, 0, 4, false, false>(x, T + q * 16, tw);
+    uni_levels<4, 0, 0, 4, false, false, UT>(x, T + q * 16, tw, gtw);
 #pragma unroll
     for (int j = 0; j < 16; ++j) io.st_out((uint32_t)(T + q * 16), j, x[j]);
 }
