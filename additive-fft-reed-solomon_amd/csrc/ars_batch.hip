@@ -147,8 +147,10 @@ __device__ __forceinline__ void run_body(const BItem &it, IO &io, int q, int p, 
         } else if constexpr (V == 2) {
             body_encode_kpow2_small<TL>(io, it.arg & 0xffffu, it.arg >> 16, tw);
         } else if constexpr (TL >= 1) {
+            // every lane runs the same transform: the twiddles' high words by
+            // scalar loads (uni_levels)
             LdsTabs tabs{s_tab + si * (1u << TL), s_er + si * (1u << TL)};
-            body_recover_small<TL>(io, tabs, tw);
+            body_recover_small<TL, IO, LdsTabs, true>(io, tabs, tw, gtw);
         }
     } else if constexpr (V == 0 || V == 1) {
         // 64+ quads: a wave holds one shard group, so the tables' high words

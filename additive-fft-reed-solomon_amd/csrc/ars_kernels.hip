@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(256) k_recover_small(uint8_t *__restrict__ dat
     __syncthreads();
     BufIO<AM> io = make_bufio<AM>(data, N * L, data, N * L, blockIdx.x * blockDim.x + threadIdx.x, L);
     LdsTabs tabs{s_tab, s_er};
-    body_recover_small<NL>(io, tabs, tw);
+    body_recover_small<NL, BufIO<AM>, LdsTabs, true>(io, tabs, tw, g_tw);
 }
 
 template <int NL, int PQ, bool AL, int NBUF>
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(256) k_recover_small_fused(uint8_t *__restrict
     __syncthreads();
     BufIO<AM> io = make_bufio<AM>(data, N * L, data, N * L, blockIdx.x * blockDim.x + threadIdx.x, L);
     LdsTabs tabs{s_tab, s_er};
-    body_recover_small<NL>(io, tabs, tw);
+    body_recover_small<NL, BufIO<AM>, LdsTabs, true>(io, tabs, tw, g_tw);
 }
 
 template <int NL, int PQ, bool AL, int NBUF>

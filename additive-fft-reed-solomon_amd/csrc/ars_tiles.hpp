@@ -64,6 +64,22 @@ struct Tile {
     }
 };
 
+// UT: the twiddle indices are wave-uniform (tiles of 64+ quads, where every
+// lane of a wave holds the same shard group q, and the small recover, where
+// every lane runs the same transform): the tables' high words come by scalar
+// loads from gtw, the global table tw was staged from (UniTw), and each lane
+// reads 8 of a table's 20 bytes from LDS.  The LDS traffic of the tiles is
+// then mostly the two exchanges (measured 8.7% faster for the 128-point
+// encode).  The small encode bodies measured 4-7% slower with it and keep
+// the LDS tables.
+template <int RL, int SH, int L0, int L1, bool INV, bool BZ, bool UT>
+__device__ __forceinline__ void uni_levels(uint32_t *x, uint32_t hb, const MulTab *tw, const MulTab *gtw) {
+    if constexpr (UT)
+        run_levels_t<RL, SH, L0, L1, INV, BZ, kBurstMuls, UniTw>(x, hb, UniTw{tw, gtw});
+    else
+        run_levels<RL, SH, L0, L1, INV, BZ>(x, hb, tw);
+}
+
 // ---------------------------------------------------------------------------
 // Encode, reference algorithm (rs/src/codec.rs:241-265): for each message
 // chunk c, acc ^= IFFT_T(chunk c, omega_{(c+1)T}); parity = FFT_T(acc, 0).
@@ -91,24 +107,9 @@ __device__ __forceinline__ void body_encode_small(IO &io, uint32_t chunks, const
 // x0 (optional): chunk 0's 16 input rows, already loaded by the caller (the
 // one-codeword kernels issue those loads before staging their tables, so the
 // two latencies overlap).
-// Phase B reads its twiddles from the LDS table like phase A.  (Scalar loads
-// from the global table, which every lane indexes alike, were 1-3% faster
-// for the 128-point encode while each v_perm_b32 took its table words where
-// they lay; with the lookups in bursts every table word must sit in a VGPR,
-// and the LDS form measured 1.5% faster.)
-// UT: the twiddle indices are wave-uniform (PQ a multiple of 64, so every
-// lane of a wave holds the same shard group q): the tables' high words come
-// by scalar loads from gtw, the global table tw was staged from (UniTw), and
-// each lane reads 8 of a table's 20 bytes from LDS.  The LDS traffic is then
-// mostly the two exchanges (measured 8.7% faster for the 128-point encode).
-template <int RL, int SH, int L0, int L1, bool INV, bool BZ, bool UT>
-__device__ __forceinline__ void uni_levels(uint32_t *x, uint32_t hb, const MulTab *tw, const MulTab *gtw) {
-    if constexpr (UT)
-        run_levels_t<RL, SH, L0, L1, INV, BZ, kBurstMuls, UniTw>(x, hb, UniTw{tw, gtw});
-    else
-        run_levels<RL, SH, L0, L1, INV, BZ>(x, hb, tw);
-}
-
+// Without UT every table comes from the LDS copy.  (Whole tables by scalar
+// loads, copied to VGPRs for the bursts, measured 1.5% slower than LDS for
+// phase B; UT copies only the two low words, and takes them from LDS.)
 template <int TL, int PQ, bool ACC, int NBUF, class IO, bool UT = false>
 __device__ __forceinline__ void body_encode_tile(IO &io, int q, int p, uint32_t *tile, uint32_t chunks,
                                                  const MulTab *tw, const uint32_t *x0 = nullptr,
@@ -216,8 +217,11 @@ __device__ __forceinline__ void body_encode_kpow2_tile(IO &io, int q, int p, uin
 // and, as the reference does (it writes the error value, rs/src/codec.rs:560-566),
 // the erased shard's previous content g_p is XORed in: out_p = c_p ^ g_p.
 // Input and output rows are the same codeword.
-template <int NL, class IO, class TABS>
-__device__ __forceinline__ void body_recover_small(IO &io, const TABS &tabs, const MulTab *tw) {
+// UT: the twiddles' high words by scalar loads (every lane runs the same
+// transform; the row tables stay in LDS, they may differ per lane in a batch)
+template <int NL, class IO, class TABS, bool UT = false>
+__device__ __forceinline__ void body_recover_small(IO &io, const TABS &tabs, const MulTab *tw,
+                                                   const MulTab *gtw = nullptr) {
     constexpr int N = 1 << NL;
     uint32_t x[N], orig[N];
 #pragma unroll
@@ -226,7 +230,7 @@ __device__ __forceinline__ void body_recover_small(IO &io, const TABS &tabs, con
         const uint32_t u = gmul4(orig[i], tabs.tab(i));
         x[i] = tabs.erased(i) ? 0u : u;
     }
-    run_levels<NL, 0, 0, NL, true, true>(x, 0u, tw);
+    uni_levels<NL, 0, 0, NL, true, true, UT>(x, 0u, tw, gtw);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         uint32_t a = 0;
@@ -235,7 +239,7 @@ __device__ __forceinline__ void body_recover_small(IO &io, const TABS &tabs, con
             if (!((i >> b) & 1)) a ^= x[i | (1 << b)];
         x[i] = a;
     }
-    run_levels<NL, 0, 0, NL, false, true>(x, 0u, tw);
+    uni_levels<NL, 0, 0, NL, false, true, UT>(x, 0u, tw, gtw);
 #pragma unroll
     for (int i = 0; i < N; ++i)
         if (tabs.erased(i)) io.st_out(0u, i, gmul4(x[i], tabs.tab(i)) ^ orig[i]);
