@@ -91,7 +91,7 @@ int ars_autotune(ars_ctx *ctx, uint32_t n, uint32_t k, size_t L, ars_tune_result
             for (int r = 0; r < kRounds; ++r) {
                 for (int i = 0; i < kNumVariants; ++i) {
                     const Variant &v = kVariants[i];
-                    if (r > 0 && t[i] < 0) continue;  // failed to launch in round 0
+                    if (r > 0 && t[i] < 0) continue;  // did not launch in the first pass
                     double us;
                     if (op == 0)
                         us = time_us([&] { return ars::launch_encode_ref(tl, true, false, buf + (size_t)T * L, buf,

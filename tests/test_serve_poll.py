@@ -56,7 +56,7 @@ def test_slot_retirement_and_thresholds(tmp_path):
 
 
 def test_hip_codes_used_by_the_server():
-    """The codes the driver above assumes are the HIP runtime's."""
+    """The HIP status codes the C test program above assumes are the runtime's."""
     hdr = "/opt/rocm/include/hip/hip_runtime_api.h"
     if not os.path.exists(hdr):
         pytest.skip("HIP headers missing")
