@@ -68,16 +68,19 @@ Pipe *pipe_for(ars_ctx *ctx) {
 
 int init_pipe(ars_ctx *ctx, Pipe *p) {
     if (p->ready) return ARS_OK;
-    hipError_t e;
+    // After a failed set-up the next call lands here again: create only what
+    // is still missing, so nothing made by the earlier attempt is leaked.
+    hipError_t e = hipSuccess;
     for (Slot &s : p->slots) {
-        if ((e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking)) != hipSuccess ||
-            (e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming)) != hipSuccess ||
-            (e = hipMalloc(&s.dev, 2 * kChunkBytes)) != hipSuccess ||
-            (e = hipHostMalloc(&s.h_in, kChunkBytes, hipHostMallocDefault)) != hipSuccess ||
-            (e = hipHostMalloc(&s.h_out, kChunkBytes, hipHostMallocDefault)) != hipSuccess)
+        if ((!s.stream && (e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking)) != hipSuccess) ||
+            (!s.done && (e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming)) != hipSuccess) ||
+            (!s.dev && (e = hipMalloc(&s.dev, 2 * kChunkBytes)) != hipSuccess) ||
+            (!s.h_in && (e = hipHostMalloc(&s.h_in, kChunkBytes, hipHostMallocDefault)) != hipSuccess) ||
+            (!s.h_out && (e = hipHostMalloc(&s.h_out, kChunkBytes, hipHostMallocDefault)) != hipSuccess))
             return ars::hip_error(e, "host pipeline set-up");
     }
-    if ((e = hipMalloc(&p->plan, sizeof(ars::RecPrep))) != hipSuccess) return ars::hip_error(e, "host pipeline plan");
+    if (!p->plan && (e = hipMalloc(&p->plan, sizeof(ars::RecPrep))) != hipSuccess)
+        return ars::hip_error(e, "host pipeline plan");
     (void)ctx;
     if (!p->pool) p->pool = new (std::nothrow) WorkPool(copy_workers());
     if (!p->pool) return ars::set_error(ARS_E_NOMEM, "host pipeline workers");
