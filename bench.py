@@ -181,6 +181,17 @@ def _profile_entry(fname, kernel):
     return kernel, recs[kernel]
 
 
+def _profiled_run_launch_us(fname, kernel):
+    """The per-launch time (HIP events) that the profiled run's own bench line
+    measured for `kernel`, on the box the profile comes from; None unless the
+    profile holds `kernel` and matches this run's device sources."""
+    if _profile_entry(fname, kernel)[0] is None:
+        return None
+    with open(os.path.join(ROOT, "profiles", fname)) as f:
+        line = json.load(f).get("bench_line") or {}
+    return line.get("launch_us") if line.get("kernel") == kernel else None
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE passes (profiles/traffic_256x128_1MiB.json), or None."""
@@ -1424,7 +1435,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                      "traffic": traffic, "algorithmic_bytes": dom_bytes, "launch_us": round(dom_s * 1e6, 2),
-                     "rocprof_avg_us": stat["avg_us"] if stat else None},
+                     "rocprof_avg_us": stat["avg_us"] if stat else None,
+                     # the same kernel timed with HIP events in the profiled run itself: rocprof's
+                     # average is compared with this, since the clock under these kernels differs
+                     # from box to box
+                     "rocprof_run_launch_us": _profiled_run_launch_us("kernel_stats_256x128_1MiB.json", dom)},
         "valu_roofline": valu_roofline(dom, dom_s, valu_rates),
         "valu_issue_rates": {k: round(v / 1e12, 3) for k, v in valu_rates.items()} if valu_rates else None,
         "encode": {"GBps": round(enc_gbps, 1), "us": round(s_enc * 1e6, 2),
