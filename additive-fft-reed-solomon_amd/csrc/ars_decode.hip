@@ -279,23 +279,27 @@ __global__ void __launch_bounds__(64) k_decode_tail(uint8_t *__restrict__ rx, ui
 
 constexpr int kCap = 264;  // coefficient slots (degrees < 257 and slack)
 
-__device__ __forceinline__ int wave_deg(const uint8_t *p) {
-    int d = -1;
-    for (int s = 0; s < kCap; s += 64) {
+// Degree of p, given that no coefficient above slot hi is non-zero (slots
+// above hi may hold stale values and are not read).  Scans 64-slot windows
+// from the top down and stops at the first one that holds a non-zero.
+__device__ __forceinline__ int wave_deg(const uint8_t *p, int hi = kCap - 1) {
+    for (int s = hi & ~63; s >= 0; s -= 64) {
         const int i = s + (int)threadIdx.x;
-        const uint64_t m = __ballot(i < kCap && p[i] != 0);
-        if (m) d = s + 63 - (int)__clzll(m);
+        const uint64_t m = __ballot(i <= hi && p[i] != 0);
+        if (m) return s + 63 - (int)__clzll(m);
     }
-    return d;
+    return -1;
 }
 
-__device__ __forceinline__ void wave_fill(uint8_t *p, uint8_t c0) {
-    for (int i = threadIdx.x; i < kCap; i += 64) p[i] = i == 0 ? c0 : (uint8_t)0;
+__device__ __forceinline__ void wave_fill(uint8_t *p, uint8_t c0, int hi = kCap - 1) {
+    for (int i = threadIdx.x; i <= hi; i += 64) p[i] = i == 0 ? c0 : (uint8_t)0;
 }
 
 // r <- r mod b, q <- r div b (deg b = db >= 0).  Returns deg of the new r.
+// q is written in slots 0 .. dr - db (the quotient's degree); each step
+// cancels r's leading term, so the new degree is searched below it.
 __device__ int wave_divmod(const WaveTabs &t, uint8_t *r, int dr, const uint8_t *b, int db, uint8_t *q) {
-    wave_fill(q, 0);
+    wave_fill(q, 0, dr - db);
     __syncthreads();
     const uint8_t binv = t.ex[255 - t.lg[b[db]]];
     for (int guard = 0; dr >= db && guard < kCap; ++guard) {
@@ -305,24 +309,28 @@ __device__ int wave_divmod(const WaveTabs &t, uint8_t *r, int dr, const uint8_t 
         if (threadIdx.x == 0) q[sh] = f;
         for (int j = threadIdx.x; j <= db; j += 64) r[sh + j] ^= lmul(t.ex, t.lg, f, b[j]);
         __syncthreads();
-        dr = wave_deg(r);
+        dr = wave_deg(r, dr - 1);
     }
     return dr;
 }
 
-// out = a * b + c (out distinct from a, b, c).  Returns deg out.
+// out = a * b + c (out distinct from a, b, c; dc = deg c).  Returns deg out.
+// Only slots up to max(dc, da + db) are read from c and written to out; the
+// callers read every polynomial only up to its degree.
 __device__ int wave_muladd(const WaveTabs &t, const uint8_t *a, int da, const uint8_t *b, int db, const uint8_t *c,
-                           uint8_t *out) {
-    for (int i = threadIdx.x; i < kCap; i += 64) {
-        uint8_t acc = c[i];
-        if (da >= 0 && db >= 0 && i <= da + db) {
+                           int dc, uint8_t *out) {
+    const int dp = (da >= 0 && db >= 0) ? da + db : -1;
+    const int hi = min(dp > dc ? dp : dc, kCap - 1);
+    for (int i = threadIdx.x; i <= hi; i += 64) {
+        uint8_t acc = i <= dc ? c[i] : (uint8_t)0;
+        if (i <= dp) {
             const int j0 = i - db > 0 ? i - db : 0, j1 = i < da ? i : da;
             for (int j = j0; j <= j1; ++j) acc ^= lmul(t.ex, t.lg, a[j], b[i - j]);
         }
         out[i] = acc;
     }
     __syncthreads();
-    return wave_deg(out);
+    return wave_deg(out, hi);
 }
 
 // The basis table xl (256 x 256) holds the monomial coefficients of X_j in
@@ -406,7 +414,7 @@ __global__ void __launch_bounds__(64) k_decode_columns(uint8_t *__restrict__ rx,
             P[S][i] = acc;
         }
         __syncthreads();
-        const int ds = wave_deg(P[S]);
+        const int ds = wave_deg(P[S], (int)T - 1);
         int dl = -1;
         if (ds >= 0) {
             // X_T = q_t s + r_t
@@ -434,14 +442,14 @@ __global__ void __launch_bounds__(64) k_decode_columns(uint8_t *__restrict__ rx,
                 dz0 = dz1;
                 z1 = zr;
                 dz1 = dr;
-                const int dnu = wave_muladd(tabs, P[Q], dq, P[u1], du1, P[u0], P[tt]);
+                const int dnu = wave_muladd(tabs, P[Q], dq, P[u1], du1, P[u0], du0, P[tt]);
                 int f = u0;
                 u0 = u1;
                 du0 = du1;
                 u1 = tt;
                 du1 = dnu;
                 tt = f;
-                const int dnv = wave_muladd(tabs, P[Q], dq, P[v1], dv1, P[v0], P[tt]);
+                const int dnv = wave_muladd(tabs, P[Q], dq, P[v1], dv1, P[v0], dv0, P[tt]);
                 f = v0;
                 v0 = v1;
                 dv0 = dv1;
@@ -449,10 +457,8 @@ __global__ void __launch_bounds__(64) k_decode_columns(uint8_t *__restrict__ rx,
                 dv1 = dnv;
                 tt = f;
             }
-            (void)du0;
-            (void)dv0;
             // lambda = u + v q_t into P[tt]
-            dl = wave_muladd(tabs, P[v1], dv1, P[QT], dqt, P[u1], P[tt]);
+            dl = wave_muladd(tabs, P[v1], dv1, P[QT], dqt, P[u1], du1, P[tt]);
             wave_to_novel(tabs, P[tt], dl, xt, P[LX]);
             wave_to_novel(tabs, P[v1], dv1, xt, P[VX]);
             // derivative in the basis X over n coefficients
