@@ -304,10 +304,14 @@ __device__ int wave_divmod(const WaveTabs &t, uint8_t *r, int dr, const uint8_t 
     const uint8_t binv = t.ex[255 - t.lg[b[db]]];
     for (int guard = 0; dr >= db && guard < kCap; ++guard) {
         const int sh = dr - db;
-        const uint8_t f = lmul(t.ex, t.lg, r[dr], binv);
+        const uint8_t f = lmul(t.ex, t.lg, r[dr], binv);  // non-zero: r[dr] is r's leading term
+        const uint32_t lf = t.lg[f];
         __syncthreads();  // every lane has read r[dr]
         if (threadIdx.x == 0) q[sh] = f;
-        for (int j = threadIdx.x; j <= db; j += 64) r[sh + j] ^= lmul(t.ex, t.lg, f, b[j]);
+        for (int j = threadIdx.x; j <= db; j += 64) {
+            const uint8_t bj = b[j];
+            if (bj) r[sh + j] ^= t.ex[lf + t.lg[bj]];
+        }
         __syncthreads();
         dr = wave_deg(r, dr - 1);
     }
@@ -316,7 +320,9 @@ __device__ int wave_divmod(const WaveTabs &t, uint8_t *r, int dr, const uint8_t 
 
 // out = a * b + c (out distinct from a, b, c; dc = deg c).  Returns deg out.
 // Only slots up to max(dc, da + db) are read from c and written to out; the
-// callers read every polynomial only up to its degree.
+// callers read every polynomial only up to its degree.  The leading
+// coefficients are non-zero, so deg out = max(da + db, dc) unless the two
+// tie (then the leading terms may cancel and the degree is searched).
 __device__ int wave_muladd(const WaveTabs &t, const uint8_t *a, int da, const uint8_t *b, int db, const uint8_t *c,
                            int dc, uint8_t *out) {
     const int dp = (da >= 0 && db >= 0) ? da + db : -1;
@@ -330,7 +336,7 @@ __device__ int wave_muladd(const WaveTabs &t, const uint8_t *a, int da, const ui
         out[i] = acc;
     }
     __syncthreads();
-    return wave_deg(out, hi);
+    return dp == dc ? wave_deg(out, hi) : hi;
 }
 
 // The basis table xl (256 x 256) holds the monomial coefficients of X_j in
