@@ -409,15 +409,27 @@ __global__ void __launch_bounds__(64) k_decode_columns(uint8_t *__restrict__ rx,
             sx[i] = v ? tabs.lg[v] : kLogZero;  // logs of the syndrome coefficients
         }
         __syncthreads();
-        // s = sum_j sx[j] X_j (row j is zero above column j, so the loop runs
-        // the same trip count on every lane)
-        for (int i = lane; i < kCap; i += 64) {
-            uint8_t acc = 0;
-            if (i < (int)T) {
+        // s = sum_j sx[j] X_j.  Row j of the basis table is zero above column
+        // j, so slot i takes only the T - i rows j >= i.  A lane computes slot
+        // i and slot T - 1 - i in one loop of T + 1 terms (the same trip count
+        // on every lane; T <= 128, so i < 64 covers every slot).
+        for (int i = lane; i < kCap; i += 64)
+            if (i >= (int)T) P[S][i] = 0;
+        if (lane < (T + 1) / 2) {
+            const int i1 = (int)lane, i2 = (int)T - 1 - i1, n1 = (int)T - i1;
+            uint8_t a1 = 0, a2 = 0;
 #pragma unroll 8
-                for (uint32_t j = 0; j < T; ++j) acc ^= lmul_log(tabs, sx[j], xt[j * 256 + i]);
+            for (int m = 0; m <= (int)T; ++m) {
+                const bool first = m < n1;
+                const int i = first ? i1 : i2, j = first ? i1 + m : i2 + (m - n1);
+                const uint8_t v = lmul_log(tabs, sx[j], xt[j * 256 + i]);
+                if (first)
+                    a1 ^= v;
+                else
+                    a2 ^= v;
             }
-            P[S][i] = acc;
+            P[S][i1] = a1;
+            if (i2 != i1) P[S][i2] = a2;
         }
         __syncthreads();
         const int ds = wave_deg(P[S], (int)T - 1);
