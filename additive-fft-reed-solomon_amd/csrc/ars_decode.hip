@@ -339,6 +339,44 @@ __device__ int wave_muladd(const WaveTabs &t, const uint8_t *a, int da, const ui
     return dp == dc ? wave_deg(out, hi) : hi;
 }
 
+// The two cofactor updates of one Euclid step, which share the quotient a:
+// out1 = a * b1 + c1 and out2 = a * b2 + c2 in one pass (one barrier, a's
+// terms read once per slot, the two chains interleaved).  out may be c (a
+// lane reads c[i] before it writes out[i], at its own slots only), not a or
+// b.  Degrees as in wave_muladd, returned in d1 and d2.
+__device__ void wave_muladd2(const WaveTabs &t, const uint8_t *a, int da, const uint8_t *b1, int db1,
+                             const uint8_t *c1, int dc1, uint8_t *out1, int &d1, const uint8_t *b2, int db2,
+                             const uint8_t *c2, int dc2, uint8_t *out2, int &d2) {
+    const int dp1 = (da >= 0 && db1 >= 0) ? da + db1 : -1, dp2 = (da >= 0 && db2 >= 0) ? da + db2 : -1;
+    const int hi1 = min(dp1 > dc1 ? dp1 : dc1, kCap - 1), hi2 = min(dp2 > dc2 ? dp2 : dc2, kCap - 1);
+    const int hi = hi1 > hi2 ? hi1 : hi2;
+    for (int i = threadIdx.x; i <= hi; i += 64) {
+        uint8_t acc1 = i <= dc1 ? c1[i] : (uint8_t)0, acc2 = i <= dc2 ? c2[i] : (uint8_t)0;
+        if (da >= 0) {
+            const int j0 = i - (db1 > db2 ? db1 : db2) > 0 ? i - (db1 > db2 ? db1 : db2) : 0, j1 = i < da ? i : da;
+            for (int j = j0; j <= j1; ++j) {
+                const uint8_t aj = a[j];
+                if (!aj) continue;
+                const uint32_t la = t.lg[aj];
+                const int k = i - j;
+                if (k <= db1) {
+                    const uint8_t x = b1[k];
+                    if (x) acc1 ^= t.ex[la + t.lg[x]];
+                }
+                if (k <= db2) {
+                    const uint8_t x = b2[k];
+                    if (x) acc2 ^= t.ex[la + t.lg[x]];
+                }
+            }
+        }
+        if (i <= hi1) out1[i] = acc1;
+        if (i <= hi2) out2[i] = acc2;
+    }
+    __syncthreads();
+    d1 = dp1 == dc1 ? wave_deg(out1, hi1) : hi1;
+    d2 = dp2 == dc2 ? wave_deg(out2, hi2) : hi2;
+}
+
 // The basis table xl (256 x 256) holds the monomial coefficients of X_j in
 // the log domain: xl[j][i] = log(coefficient of x^i in X_j), kLogZero for 0.
 constexpr uint8_t kLogZero = 255;
@@ -460,20 +498,21 @@ __global__ void __launch_bounds__(64) k_decode_columns(uint8_t *__restrict__ rx,
                 dz0 = dz1;
                 z1 = zr;
                 dz1 = dr;
-                const int dnu = wave_muladd(tabs, P[Q], dq, P[u1], du1, P[u0], du0, P[tt]);
+                // u' = q u1 + u0 and v' = q v1 + v0, each in place over u0 / v0
+                // (a lane reads c[i] and writes out[i] only at its own slots)
+                int dnu, dnv;
+                wave_muladd2(tabs, P[Q], dq, P[u1], du1, P[u0], du0, P[u0], dnu, P[v1], dv1, P[v0], dv0, P[v0],
+                             dnv);
                 int f = u0;
                 u0 = u1;
                 du0 = du1;
-                u1 = tt;
+                u1 = f;
                 du1 = dnu;
-                tt = f;
-                const int dnv = wave_muladd(tabs, P[Q], dq, P[v1], dv1, P[v0], dv0, P[tt]);
                 f = v0;
                 v0 = v1;
                 dv0 = dv1;
-                v1 = tt;
+                v1 = f;
                 dv1 = dnv;
-                tt = f;
             }
             // lambda = u + v q_t into P[tt]
             dl = wave_muladd(tabs, P[v1], dv1, P[QT], dqt, P[u1], du1, P[tt]);
