@@ -452,7 +452,7 @@ def summarize_rocprof(d):
     stamp = {"device_source_sha": sha, "commit": commit}
     wl = "(n=256,k=128), L=1 MiB, e=128"
     how = ("Run on one MI355X from /tmp with TMPDIR=/tmp, repo at $R; each counter set in a run of its own, "
-           "every pass under `timeout -s KILL 60`.  Summarised by `python bench.py --summarize-rocprof DIR`, "
+           "every pass under a `timeout -s KILL` of its own (60-90 s).  Summarised by `python bench.py --summarize-rocprof DIR`, "
            f"device sources {sha[:16]} (bench.device_source_sha), commit {commit[:12]}.")
 
     def md(fname, title, cmds, body):
