@@ -388,31 +388,30 @@ __device__ __forceinline__ uint8_t lmul_log(const WaveTabs &t, uint8_t la, uint8
 
 // Basis X coefficients of the monomial polynomial r (deg dr, destroyed) into
 // out, by back-substitution on the monic rows X_d.  Row d has zeros above
-// column d, so every lane updates its four slots of r unconditionally, and
-// row d - 1 is loaded while row d is applied.
+// column d, so every lane updates its four slots of r unconditionally.  A
+// lane's slots are 4 lane .. 4 lane + 3, so it reads each row as one dword
+// (global byte loads cost the texture unit one lane per cycle); row d - 1 is
+// loaded while row d is applied.
 __device__ void wave_to_novel(const WaveTabs &t, uint8_t *r, int dr, const uint8_t *__restrict__ xl,
                               uint8_t *out) {
     const int lane = threadIdx.x;
     wave_fill(out, 0);
     int d = dr < 255 ? dr : 255;
-    uint8_t cur[4], nxt[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) cur[k] = d >= 0 ? xl[d * 256 + lane + 64 * k] : kLogZero;
+    constexpr uint32_t kZeroRow = 0xffffffffu;  // four log-0 entries
+    uint32_t cur = d >= 0 ? *reinterpret_cast<const uint32_t *>(xl + d * 256 + 4 * lane) : kZeroRow;
     __syncthreads();
     for (; d >= 0; --d) {
         const uint8_t a = r[d];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) nxt[k] = d > 0 ? xl[(d - 1) * 256 + lane + 64 * k] : kLogZero;
+        const uint32_t nxt = d > 0 ? *reinterpret_cast<const uint32_t *>(xl + (d - 1) * 256 + 4 * lane) : kZeroRow;
         __syncthreads();
         if (a) {
             const uint8_t la = t.lg[a];
             if (lane == 0) out[d] = a;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) r[lane + 64 * k] ^= lmul_log(t, la, cur[k]);
+            for (int b = 0; b < 4; ++b) r[4 * lane + b] ^= lmul_log(t, la, (uint8_t)(cur >> (8 * b)));
         }
         __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 4; ++k) cur[k] = nxt[k];
+        cur = nxt;
     }
 }
 
@@ -447,36 +446,48 @@ __global__ void __launch_bounds__(64) k_decode_columns(uint8_t *__restrict__ rx,
             sx[i] = v ? tabs.lg[v] : kLogZero;  // logs of the syndrome coefficients
         }
         __syncthreads();
-        // s = sum_j sx[j] X_j.  Row j of the basis table is zero above column
-        // j, so slot i takes only the T - i rows j >= i.  A lane computes slot
-        // i and slot T - 1 - i in one loop of T + 1 terms (the same trip count
-        // on every lane; T <= 128, so i < 64 covers every slot).
-        for (int i = lane; i < kCap; i += 64)
-            if (i >= (int)T) P[S][i] = 0;
-        if (lane < (T + 1) / 2) {
-            const int i1 = (int)lane, i2 = (int)T - 1 - i1, n1 = (int)T - i1;
-            uint8_t a1 = 0, a2 = 0;
-#pragma unroll 8
-            for (int m = 0; m <= (int)T; ++m) {
-                const bool first = m < n1;
-                const int i = first ? i1 : i2, j = first ? i1 + m : i2 + (m - n1);
-                const uint8_t v = lmul_log(tabs, sx[j], xt[j * 256 + i]);
-                if (first)
-                    a1 ^= v;
-                else
-                    a2 ^= v;
+        // s = sum_j sx[j] X_j.  Global byte loads cost the texture unit one lane
+        // per cycle, so the basis rows are read as dwords: a lane takes the 4
+        // adjacent slots 4 (lane mod 32) .. + 3 of every other row (the two
+        // half-waves split the rows and add their partial sums at the end).
+        // Row j is zero above column j, so slots >= T sum to 0.
+        {
+            const int half = (int)(lane >> 5), q4 = (int)(lane & 31u) * 4;
+            uint32_t acc = 0;
+            if (q4 < (int)T) {
+                for (int j = half; j < (int)T; j += 2) {
+                    const uint32_t la = sx[j];
+                    const uint32_t w = *reinterpret_cast<const uint32_t *>(xt + j * 256 + q4);
+                    if (la != kLogZero) {
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) {
+                            const uint32_t lx = (w >> (8 * b)) & 0xffu;
+                            if (lx != kLogZero) acc ^= (uint32_t)tabs.ex[la + lx] << (8 * b);
+                        }
+                    }
+                }
             }
-            P[S][i1] = a1;
-            if (i2 != i1) P[S][i2] = a2;
+            acc ^= (uint32_t)__shfl_xor((int)acc, 32);
+            if (half == 0) {
+#pragma unroll
+                for (int b = 0; b < 4; ++b) P[S][q4 + b] = (uint8_t)(acc >> (8 * b));
+            }
+            for (int i = 128 + (int)lane; i < kCap; i += 64) P[S][i] = 0;
         }
         __syncthreads();
         const int ds = wave_deg(P[S], (int)T - 1);
         int dl = -1;
         if (ds >= 0) {
             // X_T = q_t s + r_t
-            for (int i = lane; i < kCap; i += 64) {
-                const uint8_t l = i <= (int)T ? xt[T * 256 + i] : kLogZero;
-                P[R][i] = l == kLogZero ? (uint8_t)0 : tabs.ex[l];
+            {
+                // row T as dwords (slots 0 .. 255; entries above column T are log 0)
+                const uint32_t w = *reinterpret_cast<const uint32_t *>(xt + T * 256 + 4 * lane);
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const uint32_t l = (w >> (8 * b)) & 0xffu;
+                    P[R][4 * lane + b] = l == kLogZero ? (uint8_t)0 : tabs.ex[l];
+                }
+                for (int i = 256 + (int)lane; i < kCap; i += 64) P[R][i] = 0;
             }
             __syncthreads();
             const int drt = wave_divmod(tabs, P[R], (int)T, P[S], ds, P[QT]);
